@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Throughput of the MPM snow substep on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Workload: the 4,194,304-particle snow slab on a 256^3 grid (dx = 1/256,
+4x4x4 node blocks), per-particle E0 in [1e5, 2e5], dt = 2.5e-5 — the
+configuration the headline metric is quoted on.  A step is one substep over
+all particles (binning, stress + F update + P2G, grid update + boundary
+conditions, G2P + advection), inputs resident in HBM.  Prints one JSON line.
+
+roofline: the dominant kernel's algorithmic bytes per launch (per-particle
+bytes x particles, see ALG_BYTES) over its mean launch time, measured with
+HIP events on the library's stream around every launch in the timed region.
+cpu_baseline: the oracle (a CPU restatement of the reference path, single
+thread like the reference) timed on this host for one substep of the same
+scene.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "realtime-deformations_amd", "python"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes_per_particle(kernel: str, a_over_n: float) -> float:
+    """Algorithmic HBM bytes per particle of one launch of `kernel`.
+
+    Particle state: x, v (12 B each), B, F_E, F_P (36 B each), constants
+    m, V0, E0 (4 B each); grid: m + p (16 B) written by P2G and read by the
+    grid update, v (12 B) written by the grid update and read by G2P, per
+    active node (A of them for N particles).
+    """
+    if kernel == "p2g":  # read x v B F_E F_P m V0 E0, write F_E F_P, write m+p
+        return (12 + 12 + 36 * 3 + 12) + 72 + 16 * a_over_n
+    if kernel == "g2p":  # read x and grid v, write x v B
+        return 12 + 12 * a_over_n + (12 + 12 + 36)
+    if kernel == "grid":  # read m+p, write v
+        return (16 + 12) * a_over_n
+    if kernel == "bin":  # read x, write keys + index
+        return 12 + 8
+    raise KeyError(kernel)
+
+
+def substep_alg_bytes(a_over_n: float) -> float:
+    """B_alg = 2(12+12+36+36+36) + 4*3 + 56 A/N per particle-substep."""
+    return 2 * (12 + 12 + 36 + 36 + 36) + 4 * 3 + 56 * a_over_n
+
+
+def cpu_baseline(sc, V0, threads=1):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    prm = sc["params"]
+    p = O.make_params(sc["dims"], sc["h"], compose=O.LITERAL, bc_mode=prm["bc_mode"], E=prm["E"], nu=prm["nu"],
+                      xi=prm["xi"], theta_c=prm["theta_c"], theta_s=prm["theta_s"], threads=threads)
+    n = sc["x"].shape[0]
+    s = O.Sim(p, n)
+    s.set_particles(sc["x"], sc["v"], sc["m"], sc["E0"], V0=V0)
+    t0 = time.perf_counter()
+    s.substeps(sc["dt"], 1)
+    dt = time.perf_counter() - t0
+    return dict(value=n / dt, unit="particle-substeps/s", cores=threads, kind="port",
+                sample=f"1 substep of the same {n}-particle slab on the 256^3 grid, oracle FP32, "
+                       f"{threads} thread(s), {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); "
+              "multi-GPU sharding of the substep is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import mpm_amd as M
+    from mpm_amd import scenes
+
+    sc = scenes.slab()
+    n = sc["x"].shape[0]
+    prm = sc["params"]
+    p = M.default_params(sc["dims"], sc["h"], compose=M.COMPOSE_LITERAL, bc_mode=prm["bc_mode"], E=prm["E"],
+                         nu=prm["nu"], xi=prm["xi"], theta_c=prm["theta_c"], theta_s=prm["theta_s"])
+    sim = M.Simulation(p, n)
+    sim.set_particles(sc["x"], sc["v"], sc["m"], sc["E0"])
+    sim.init_volumes()
+    V0 = sim.particles()["V0"] if not args.no_cpu_baseline else None
+    dt = sc["dt"]
+
+    sim.substeps(dt, args.warmup)
+    sim.sync()
+    sim.profile_begin()
+    sim.substeps(dt, args.steps)
+    total_ms, kern = sim.profile_end()
+    sim.sync()
+    active = sim.count_active()
+    a_over_n = active / n
+
+    ms_per_step = total_ms / args.steps
+    value = n * args.steps / (total_ms / 1e3)
+    dom = max(kern, key=lambda k: kern[k][0])
+    dom_ms = kern[dom][0] / max(1, kern[dom][1])
+    dom_bytes = alg_bytes_per_particle(dom, a_over_n) * n
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    sub_bytes = substep_alg_bytes(a_over_n) * n
+    out = {
+        "metric": "particle-substeps/s at 4M particles (% HBM roofline); fwd+bwd substeps/s",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "snow slab 4,194,304 particles, 256^3 grid, 4^3 blocks, dt 2.5e-5, E0 U[1e5,2e5]",
+                   "particles": n, "grid": list(sc["dims"]), "active_nodes": active, "compose": "literal"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_particle": alg_bytes_per_particle(dom, a_over_n),
+                     "mean_launch_ms": dom_ms},
+        "substep_roofline": {"alg_bytes_per_particle": substep_alg_bytes(a_over_n),
+                             "achieved": sub_bytes / (ms_per_step / 1e3) / 1e9,
+                             "frac": sub_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS},
+        "kernels_ms": {k: (v[0] / max(1, v[1])) for k, v in kern.items() if v[1]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sc, V0, args.cpu_threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

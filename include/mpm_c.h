@@ -1,0 +1,165 @@
+/*
+ * mpm_c.h — C ABI of the MI355X (gfx950) MPM snow substep library
+ * (libmpm_hip.so).  Plain pointers, sizes and status codes; no C++ or torch
+ * types cross it.  It replaces the simulation-step surface of the reference's
+ * MaterialPointMethod::LagrangeEulerView
+ * (realtime-deformations/material_point_method.hpp:174-200), which
+ * include/mpm/material_point_method.hpp re-exposes on top of these calls.
+ *
+ * Conventions
+ *  - Every function returns MPM_OK (0) or an MPM_ERR_* code; the message of
+ *    the last failure is mpm_last_error(ctx).  No exception crosses the ABI.
+ *  - Host arrays are caller-owned and copied synchronously; the context owns
+ *    all device memory, one HIP stream and its HIP graphs, and keeps no caller
+ *    pointer.  A context is single-device and externally serialised.
+ *  - Vectors are 3 floats per particle (x, y, z); matrices are 9 floats per
+ *    particle in glm's column-major order (element (row r, col c) at 3c + r),
+ *    the layout of the reference's glm::mat3 members.
+ *  - Grid arrays are dense over the nodes, index i*J*K + j*K + k
+ *    (realtime-deformations/material_point_method.hpp:129-131).
+ */
+#ifndef MPM_C_H
+#define MPM_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpm_ctx mpm_ctx;
+
+enum {
+  MPM_OK = 0,
+  MPM_ERR_ARG = 1,       /* invalid argument                                   */
+  MPM_ERR_HIP = 2,       /* HIP runtime failure (message has the HIP error)    */
+  MPM_ERR_BOUNDS = 3,    /* a particle's 5^3 stencil leaves the grid           */
+  MPM_ERR_NONFINITE = 4, /* the SVD met a non-finite F (reference: SVD Error)  */
+  MPM_ERR_STATE = 5,     /* call out of order (e.g. no particles set)          */
+  MPM_ERR_NOMEM = 6      /* device allocation failed                           */
+};
+
+/* F_E recomposition after the singular-value clamp
+ * (realtime-deformations/material_point_method.cpp:323-326). */
+enum {
+  MPM_COMPOSE_LITERAL = 0, /* the reference's U^T S V of the SVD of F_hat^T    */
+  MPM_COMPOSE_TEXTBOOK = 1 /* V S U^T, the physically correct recomposition    */
+};
+
+/* Grid boundary handling (bit mask). */
+enum {
+  MPM_BC_COLLIDERS = 1, /* SDF box colliders, material_point_method.cpp:264-304 */
+  MPM_BC_DOMAIN = 2     /* sticky floor band j<3, separating walls elsewhere    */
+};
+
+/* Stages of one substep, in the reference caller's order
+ * (realtime-deformations/main.cpp:192-218). */
+enum {
+  MPM_STAGE_P2G = 0,     /* rasterizeParticlesToGrid            cpp:94-129  */
+  MPM_STAGE_VOLUMES = 1, /* computeParticleVolumesAndDensities  cpp:131-142 */
+  MPM_STAGE_FORCES = 2,  /* computeExplicitGridForces           cpp:235-254 */
+  MPM_STAGE_GRID = 3,    /* gridVelocitiesUpdate                cpp:256-262 */
+  MPM_STAGE_COLLIDE = 4, /* gridBasedCollisions                 cpp:298-304 */
+  MPM_STAGE_FUPDATE = 5, /* updateDeformationGradient           cpp:306-330 */
+  MPM_STAGE_G2P = 6,     /* updateParticleVelocities            cpp:332-342 */
+  MPM_STAGE_ADVECT = 7   /* updateParticlePositions             cpp:344-350 */
+};
+
+typedef struct mpm_params {
+  int32_t dims[3];   /* grid nodes per axis: the ctor's max_i, max_j, max_k     */
+  int32_t compose;   /* MPM_COMPOSE_*                                           */
+  int32_t bc_mode;   /* MPM_BC_* mask                                           */
+  int32_t flags;     /* reserved, 0                                             */
+  float h;           /* grid spacing: the reference's global WeightCalculator::h*/
+  float E;           /* Young's modulus when no per-particle E0 is given (1.4e5)*/
+  float nu;          /* Poisson ratio (0.2)                                     */
+  float xi;          /* hardening coefficient (10)                              */
+  double theta_c;    /* clamp 1 - theta_c (2.5e-2)                              */
+  double theta_s;    /* clamp 1 + theta_s (5e-3 in the reference)              */
+  float gravity[3];  /* (0, -9.8, 0)                                            */
+  float svd_gap_fp64;/* TEXTBOOK only: FP64 re-solve when the smallest gap
+                        between singular values is below this times the
+                        largest; 0 disables                                    */
+  int32_t resort_every; /* fused path: re-bin particles every this many
+                        substeps (0: 16).  Results do not depend on it: a
+                        particle that left its block's halo margin takes a
+                        global-memory path until the next sort.              */
+} mpm_params;
+
+/* Oriented box collider: the pose the reference reads from Object3D
+ * (realtime-deformations/mesh.hpp:11-24) and MeshCollider::velocity
+ * (realtime-deformations/material_point_method.hpp:74-93). */
+typedef struct mpm_collider {
+  float translation[3];
+  float rotation[4]; /* quaternion w, x, y, z                                   */
+  float scale[3];    /* half extents                                            */
+  float velocity[3];
+} mpm_collider;
+
+/* Fill p with the reference's constants for a grid of dims and spacing h. */
+void mpm_default_params(mpm_params* p, int32_t ni, int32_t nj, int32_t nk, float h);
+
+/* LagrangeEulerView(max_i, max_j, max_k, particlesNum)
+ * (realtime-deformations/material_point_method.cpp:144-154). */
+int mpm_create(const mpm_params* p, int32_t n_particles, mpm_ctx** out);
+int mpm_destroy(mpm_ctx* ctx);
+const char* mpm_last_error(const mpm_ctx* ctx);
+int32_t mpm_num_particles(const mpm_ctx* ctx);
+
+/* Particle positions, velocities, masses and optional per-particle Young's
+ * moduli (NULL: params E); F_E = F_P = I, B = 0, V0 = 0, as
+ * initializeParticles leaves them (material_point_method.cpp:18-63). */
+int mpm_set_particles(mpm_ctx* ctx, const float* x, const float* v, const float* m, const float* E0);
+/* Overwrite any of B, F_E, F_P (9 floats each) and V0; NULL leaves a field. */
+int mpm_set_state(mpm_ctx* ctx, const float* B, const float* FE, const float* FP, const float* V0);
+int mpm_set_colliders(mpm_ctx* ctx, const mpm_collider* c, int32_t k);
+
+/* rasterizeParticlesToGrid + computeParticleVolumesAndDensities
+ * (realtime-deformations/main.cpp:53-54). */
+int mpm_init_volumes(mpm_ctx* ctx);
+
+/* One legacy stage (MPM_STAGE_*), exactly the reference method's semantics. */
+int mpm_stage(mpm_ctx* ctx, int32_t stage, float dt);
+
+/* nsteps substeps in the reference's order on the fused, binned path
+ * (main.cpp:192-218 repeated).  Asynchronous on the context stream; errors
+ * raised by the device are reported by the next synchronising call. */
+int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps);
+int mpm_sync(mpm_ctx* ctx);
+
+/* Readback into host arrays (NULL skips a field).  getParticles()
+ * (material_point_method.hpp:181-183). */
+int mpm_get_particles(mpm_ctx* ctx, float* x, float* v, float* B, float* FE, float* FP, float* V0);
+/* Dense grid: mass, velocity (3), force (3, stage path only), active flag. */
+int mpm_get_grid(mpm_ctx* ctx, float* m, float* vel, float* force, uint8_t* active);
+
+/* ---- measurement (no reference counterpart: the reference times stages
+ * with millisecond std::chrono stamps, realtime-deformations/utils.h:138-154,
+ * main.cpp:192-225) ----------------------------------------------------- */
+enum {
+  MPM_KERNEL_P2G = 0,  /* fused stress + F update + P2G scatter            */
+  MPM_KERNEL_GRID = 1, /* grid velocity update + boundary conditions        */
+  MPM_KERNEL_G2P = 2,  /* G2P + advection                                   */
+  MPM_KERNEL_BIN = 3,  /* binning: keys, radix sort, reorder                */
+  MPM_NKERNELS = 8
+};
+/* While profiling, every fused-path launch is bracketed by HIP events on the
+ * context stream; mpm_profile_end synchronises and returns, per MPM_KERNEL_*
+ * id, the summed kernel milliseconds and launch counts, and the wall time of
+ * the whole profiled region (first to last event). */
+int mpm_profile_begin(mpm_ctx* ctx);
+int mpm_profile_end(mpm_ctx* ctx, double* kernel_ms, int32_t* launches, double* total_ms);
+/* Number of grid nodes with non-zero mass after the last grid update. */
+int mpm_count_active(mpm_ctx* ctx, int64_t* active_nodes);
+
+/* Binning check (no reference counterpart): bins the current particles as
+ * the fused path does and returns, for n particles in their current device
+ * order, the sorted keys (block * 64 + cell-in-block), the stable sort
+ * permutation (sorted position -> current slot) and, after the round-robin
+ * reorder, the caller index held by every slot.  Any output may be NULL. */
+int mpm_debug_bin(mpm_ctx* ctx, uint32_t* sorted_keys, uint32_t* perm, int32_t* slot_caller_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPM_C_H */

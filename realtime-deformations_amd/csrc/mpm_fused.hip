@@ -1,0 +1,416 @@
+// Fused, binned substep kernels of libmpm_hip (gfx950).
+//
+// The reference runs seven passes per substep (main.cpp:192-218).  Because
+// the F update consumes the APIC matrix of the PREVIOUS G2P (it runs before
+// updateParticleVelocities, cpp:306-330), the G2P of substep n-1 and
+// everything particle-side of substep n fuse into one pass per particle:
+//
+//   G2P from grid n-1 (v, B; cpp:332-342) -> advection and clamp
+//   (cpp:344-350,381-388) -> snow stress from F^n (cpp:235-254) -> F update
+//   F^n -> F^{n+1} with B (cpp:306-330) -> scatter of mass and of the APIC
+//   momentum plus the explicit force impulse,
+//   m v + (m B D^-1 - dt V0 D^-1 P F^T)(x_i - x_p), into grid n (cpp:94-129)
+//
+// so v and B never leave registers inside a run of substeps.  One workgroup
+// processes one 4x4x4-cell home block at a time (persistent loop over the
+// block list built by the binning pass): the grid velocities around the block
+// and the accumulators it writes live in LDS halos of 9x9x9 nodes (the block,
+// the cubic stencil, and a one-cell margin for particles that drifted since
+// the last sort).  Halo strides (112, 12, 1) make a 4x4x4 cube of lanes —
+// what the round-robin particle layout produces — hit 64 distinct banks for
+// any offset.  A particle outside the margin takes a global-memory path.
+// The grid pass then turns accumulators into velocities on the active
+// blocks only.
+#include "mpm_fused.hpp"
+
+namespace mpm {
+namespace {
+
+constexpr int TPB = 256;
+constexpr int HN = 9;
+constexpr int SX = 112, SY = 12;
+constexpr int PLANE = 1008;  // >= 8*SX + 8*SY + 8 + 1
+
+enum Mode { MODE_P2G = 0, MODE_FUSED = 1, MODE_G2P = 2 };
+
+__device__ __forceinline__ float ld(const KArgs& a, int f, int p) { return a.P[(int64_t)f * a.stride + p]; }
+__device__ __forceinline__ void st(const KArgs& a, int f, int p, float v) { a.P[(int64_t)f * a.stride + p] = v; }
+
+__device__ __forceinline__ void mark_block(const BlockArgs& ba, int bi, int bj, int bk) {
+  const KArgs& a = ba.a;
+  if (bi < 0 || bj < 0 || bk < 0 || bi >= a.nb[0] || bj >= a.nb[1] || bk >= a.nb[2]) return;
+  const int gb = (bi * a.nb[1] + bj) * a.nb[2] + bk;
+  if (atomicExch(&ba.gflag[gb], ba.step) != ba.step) {
+    const int slot = atomicAdd(ba.gcnt_cur, 1);
+    ba.glist_cur[slot] = gb;
+  }
+}
+
+// Separable G2P accumulation over the 4x4x4 stencil (cpp:332-342):
+// v = sum_a wx T_a and the columns of B = sum w v d^T from partial sums
+// S (over z) and T (over y).  `fetch(ia, ib, ic)` returns the node velocity.
+template <typename Fetch>
+__device__ __forceinline__ void g2p_accum(const Axis (&ax)[3], const float (&dd)[3][4], Fetch fetch, F3& v, M3& B) {
+  float vx = 0.f, vy = 0.f, vz = 0.f;
+  float bxx = 0.f, bxy = 0.f, bxz = 0.f, byx = 0.f, byy = 0.f, byz = 0.f, bzx = 0.f, bzy = 0.f, bzz = 0.f;
+#pragma unroll
+  for (int ia = 0; ia < 4; ++ia) {
+    float tx = 0.f, ty = 0.f, tz = 0.f, tyx = 0.f, tyy = 0.f, tyz = 0.f, tzx = 0.f, tzy = 0.f, tzz = 0.f;
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) {
+      float sx = 0.f, sy = 0.f, sz = 0.f, szx = 0.f, szy = 0.f, szz = 0.f;
+#pragma unroll
+      for (int ic = 0; ic < 4; ++ic) {
+        const F3 g = fetch(ia, ib, ic);
+        const float wz = ax[2].w[ic], wzd = wz * dd[2][ic];
+        sx = fmaf(wz, g.x, sx);
+        sy = fmaf(wz, g.y, sy);
+        sz = fmaf(wz, g.z, sz);
+        szx = fmaf(wzd, g.x, szx);
+        szy = fmaf(wzd, g.y, szy);
+        szz = fmaf(wzd, g.z, szz);
+      }
+      const float wy = ax[1].w[ib], wyd = wy * dd[1][ib];
+      tx = fmaf(wy, sx, tx);
+      ty = fmaf(wy, sy, ty);
+      tz = fmaf(wy, sz, tz);
+      tyx = fmaf(wyd, sx, tyx);
+      tyy = fmaf(wyd, sy, tyy);
+      tyz = fmaf(wyd, sz, tyz);
+      tzx = fmaf(wy, szx, tzx);
+      tzy = fmaf(wy, szy, tzy);
+      tzz = fmaf(wy, szz, tzz);
+    }
+    const float wx = ax[0].w[ia], wxd = wx * dd[0][ia];
+    vx = fmaf(wx, tx, vx);
+    vy = fmaf(wx, ty, vy);
+    vz = fmaf(wx, tz, vz);
+    bxx = fmaf(wxd, tx, bxx);
+    bxy = fmaf(wxd, ty, bxy);
+    bxz = fmaf(wxd, tz, bxz);
+    byx = fmaf(wx, tyx, byx);
+    byy = fmaf(wx, tyy, byy);
+    byz = fmaf(wx, tyz, byz);
+    bzx = fmaf(wx, tzx, bzx);
+    bzy = fmaf(wx, tzy, bzy);
+    bzz = fmaf(wx, tzz, bzz);
+  }
+  v = f3(vx, vy, vz);
+  // B = sum w v d^T : column j holds sum w v d_j (glm B.c[col][row])
+  B.c[0][0] = bxx;
+  B.c[0][1] = bxy;
+  B.c[0][2] = bxz;
+  B.c[1][0] = byx;
+  B.c[1][1] = byy;
+  B.c[1][2] = byz;
+  B.c[2][0] = bzx;
+  B.c[2][1] = bzy;
+  B.c[2][2] = bzz;
+}
+
+struct StencilInfo {
+  Axis ax[3];
+  int lb[3];   // stencil origin inside the halo
+  bool fits;   // stencil inside the halo
+};
+__device__ __forceinline__ StencilInfo stencil_at(const KArgs& a, const int o[3], F3 x) {
+  StencilInfo s;
+  s.ax[0] = axis_weights(x.x, a.h, 1, a.dims[0] - 3);
+  s.ax[1] = axis_weights(x.y, a.h, 1, a.dims[1] - 3);
+  s.ax[2] = axis_weights(x.z, a.h, 1, a.dims[2] - 3);
+  s.fits = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    s.lb[d] = s.ax[d].base - 1 - o[d];
+    s.fits = s.fits && s.lb[d] >= 0 && s.lb[d] + 3 < HN;
+  }
+  return s;
+}
+
+// G2P of one particle from the LDS halo, or from global memory when its
+// stencil left the halo.
+__device__ __forceinline__ void g2p_one(const BlockArgs& ba, const float* __restrict__ vh, const int o[3], F3 x,
+                                        F3& v, M3& B) {
+  const KArgs& a = ba.a;
+  const StencilInfo s = stencil_at(a, o, x);
+  float dd[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float xc = d == 0 ? x.x : (d == 1 ? x.y : x.z);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) dd[d][k] = (float)(s.ax[d].base - 1 + k) * a.h - xc;
+  }
+  if (s.fits) {
+    const int q0 = s.lb[0] * SX + s.lb[1] * SY + s.lb[2];
+    g2p_accum(s.ax, dd, [&](int ia, int ib, int ic) {
+      const int q = q0 + ia * SX + ib * SY + ic;
+      return f3(vh[q], vh[PLANE + q], vh[2 * PLANE + q]);
+    }, v, B);
+  } else {
+    g2p_accum(s.ax, dd, [&](int ia, int ib, int ic) {
+      const float4 g = a.gvel[node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, a.nb)];
+      return f3(g.x, g.y, g.z);
+    }, v, B);
+  }
+}
+
+// P2G scatter of mass and of m v + Q (x_i - x_p) over the 4x4x4 stencil
+// (cpp:99-117 and cpp:249-252 combined); `add(ia, ib, ic, m, px, py, pz)`.
+template <typename Add>
+__device__ __forceinline__ void p2g_scatter(const KArgs& a, const StencilInfo& s, F3 x, float m, F3 mv, const M3& Q,
+                                            Add add) {
+#pragma unroll
+  for (int ia = 0; ia < 4; ++ia) {
+    const float ddx = (float)(s.ax[0].base - 1 + ia) * a.h - x.x;
+    const float wa = s.ax[0].w[ia];
+    const float qax = fmaf(Q.c[0][0], ddx, mv.x), qay = fmaf(Q.c[0][1], ddx, mv.y), qaz = fmaf(Q.c[0][2], ddx, mv.z);
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) {
+      const float ddy = (float)(s.ax[1].base - 1 + ib) * a.h - x.y;
+      const float wxy = wa * s.ax[1].w[ib];
+      const float qx = fmaf(Q.c[1][0], ddy, qax);
+      const float qy = fmaf(Q.c[1][1], ddy, qay);
+      const float qz = fmaf(Q.c[1][2], ddy, qaz);
+#pragma unroll
+      for (int ic = 0; ic < 4; ++ic) {
+        const float ddz = (float)(s.ax[2].base - 1 + ic) * a.h - x.z;
+        const float w = wxy * s.ax[2].w[ic];
+        add(ia, ib, ic, m * w, w * fmaf(Q.c[2][0], ddz, qx), w * fmaf(Q.c[2][1], ddz, qy),
+            w * fmaf(Q.c[2][2], ddz, qz));
+      }
+    }
+  }
+}
+
+// Stress from F^n, F update F^n -> F^{n+1} with B, then the P2G scatter.
+template <int COMPOSE>
+__device__ __forceinline__ void p2g_one(const BlockArgs& ba, double* __restrict__ ah, const int o[3], int p, F3 x,
+                                        F3 v, const M3& B) {
+  const KArgs& a = ba.a;
+  M3 Q;
+  float m;
+  {
+    M3 FE, FP;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        FE.c[c][r] = ld(a, FFE + 3 * c + r, p);
+        FP.c[c][r] = ld(a, FFP + 3 * c + r, p);
+      }
+    m = ld(a, FM, p);
+    const float V0 = ld(a, FVOL, p);
+    const float E = ld(a, FE0, p);
+    const M3 A = stress_matrix(E, a.nu, a.xi, V0, a.Dinv, FE, FP);
+    const M3 BD = mmul(B, a.Dinv);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) Q.c[c][r] = fmaf(m, BD.c[c][r], -a.dt * A.c[c][r]);
+    if (f_update_t<COMPOSE>(a.dt, a.Dinv, a.lo, a.hi, a.gap64, B, FE, FP)) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          st(a, FFE + 3 * c + r, p, FE.c[c][r]);
+          st(a, FFP + 3 * c + r, p, FP.c[c][r]);
+        }
+    } else {
+      atomicOr(a.status, ST_NONFINITE);
+    }
+  }
+  const F3 mv = v * m;
+  const StencilInfo s = stencil_at(a, o, x);
+  if (s.fits) {
+    const int q0 = s.lb[0] * SX + s.lb[1] * SY + s.lb[2];
+    p2g_scatter(a, s, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
+      const int q = q0 + ia * SX + ib * SY + ic;
+      atomicAdd(&ah[q], (double)vm);
+      atomicAdd(&ah[PLANE + q], (double)px);
+      atomicAdd(&ah[2 * PLANE + q], (double)py);
+      atomicAdd(&ah[3 * PLANE + q], (double)pz);
+    });
+  } else {
+    // stencil blocks: base-1 .. base+2 spans at most two blocks per axis
+    for (int di = 0; di < 2; ++di)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int dk = 0; dk < 2; ++dk)
+          mark_block(ba, (s.ax[0].base - 1 + 3 * di) >> 2, (s.ax[1].base - 1 + 3 * dj) >> 2,
+                     (s.ax[2].base - 1 + 3 * dk) >> 2);
+    p2g_scatter(a, s, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
+      float* g = reinterpret_cast<float*>(
+          a.gacc + node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, a.nb));
+      atomicAdd(g + 0, vm);
+      atomicAdd(g + 1, px);
+      atomicAdd(g + 2, py);
+      atomicAdd(g + 3, pz);
+    });
+  }
+}
+
+__device__ __forceinline__ F3 advect(const KArgs& a, F3 x, F3 v) {
+  x = x + v * a.dt;
+  const float lo = 3.0f * a.h;
+  x.x = fminf(fmaxf(x.x, lo), (float)(a.dims[0] - 3) * a.h);
+  x.y = fminf(fmaxf(x.y, lo), (float)(a.dims[1] - 3) * a.h);
+  x.z = fminf(fmaxf(x.z, lo), (float)(a.dims[2] - 3) * a.h);
+  return x;
+}
+
+template <int MODE, int COMPOSE> __global__ void __launch_bounds__(TPB) k_block_pass(BlockArgs ba) {
+  constexpr bool kG2P = MODE != MODE_P2G;
+  constexpr bool kP2G = MODE != MODE_G2P;
+  __shared__ float vh[kG2P ? 3 * PLANE : 1];
+  __shared__ double ah[kP2G ? 4 * PLANE : 1];
+  const KArgs& a = ba.a;
+  const int t = threadIdx.x;
+  const int nblocks = *ba.nblk;
+  for (int s = blockIdx.x; s < nblocks; s += gridDim.x) {
+    const int4 e = ba.blist[s];
+    const int blk = e.x, start = e.y, end = e.z;
+    const int bk = blk % a.nb[2];
+    const int tt = blk / a.nb[2];
+    const int bj = tt % a.nb[1];
+    const int bi = tt / a.nb[1];
+    const int o[3] = {4 * bi - 2, 4 * bj - 2, 4 * bk - 2};
+    if (kG2P) {
+      for (int q = t; q < HN * HN * HN; q += TPB) {
+        const int hx = q / (HN * HN), hy = (q / HN) % HN, hz = q % HN;
+        const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
+        float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (gi >= 0 && gj >= 0 && gk >= 0 && gi < a.dims[0] && gj < a.dims[1] && gk < a.dims[2])
+          g = a.gvel[node_index(gi, gj, gk, a.nb)];
+        const int l = hx * SX + hy * SY + hz;
+        vh[l] = g.x;
+        vh[PLANE + l] = g.y;
+        vh[2 * PLANE + l] = g.z;
+      }
+    }
+    if (kP2G) {
+      for (int q = t; q < 4 * PLANE; q += TPB) ah[q] = 0.0;
+    }
+    __syncthreads();
+    for (int p = start + t; p < end; p += TPB) {
+      F3 x = f3(ld(a, FX, p), ld(a, FX + 1, p), ld(a, FX + 2, p));
+      F3 v;
+      M3 B;
+      if (kG2P) {
+        g2p_one(ba, vh, o, x, v, B);
+        x = advect(a, x, v);
+        st(a, FX, p, x.x);
+        st(a, FX + 1, p, x.y);
+        st(a, FX + 2, p, x.z);
+        if (MODE == MODE_G2P) {
+          st(a, FV, p, v.x);
+          st(a, FV + 1, p, v.y);
+          st(a, FV + 2, p, v.z);
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int r = 0; r < 3; ++r) st(a, FB + 3 * c + r, p, B.c[c][r]);
+        }
+      } else {
+        v = f3(ld(a, FV, p), ld(a, FV + 1, p), ld(a, FV + 2, p));
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) B.c[c][r] = ld(a, FB + 3 * c + r, p);
+      }
+      if (kP2G) p2g_one<COMPOSE>(ba, ah, o, p, x, v, B);
+    }
+    __syncthreads();
+    if (kP2G) {
+      for (int q = t; q < HN * HN * HN; q += TPB) {
+        const int hx = q / (HN * HN), hy = (q / HN) % HN, hz = q % HN;
+        const int l = hx * SX + hy * SY + hz;
+        const float m = (float)ah[l], px = (float)ah[PLANE + l], py = (float)ah[2 * PLANE + l], pz = (float)ah[3 * PLANE + l];
+        if (m != 0.0f || px != 0.0f || py != 0.0f || pz != 0.0f) {
+          const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
+          if (gi >= 0 && gj >= 0 && gk >= 0 && gi < a.dims[0] && gj < a.dims[1] && gk < a.dims[2]) {
+            float* g = reinterpret_cast<float*>(a.gacc + node_index(gi, gj, gk, a.nb));
+            atomicAdd(g + 0, m);
+            atomicAdd(g + 1, px);
+            atomicAdd(g + 2, py);
+            atomicAdd(g + 3, pz);
+          }
+        }
+      }
+      if (t < 27) mark_block(ba, bi - 1 + t / 9, bj - 1 + (t / 3) % 3, bk - 1 + t % 3);
+    }
+    __syncthreads();
+  }
+}
+
+// v = p / m + dt g on the nodes of active blocks, boundary conditions, and
+// the accumulator reset; blocks active last substep but not now are zeroed.
+__global__ void __launch_bounds__(TPB) k_grid_active(BlockArgs ba) {
+  const KArgs& a = ba.a;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ba.gcnt_next = 0;
+  const int ncur = *ba.gcnt_cur;
+  const int nprev = *ba.gcnt_prev;
+  for (int s = blockIdx.x * 4 + wave; s < ncur + nprev; s += gridDim.x * 4) {
+    if (s < ncur) {
+      const int gb = ba.glist_cur[s];
+      const int64_t g = (int64_t)gb * 64 + lane;
+      const float4 acc = a.gacc[g];
+      float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (acc.x != 0.0f) {
+        F3 v = f3(acc.y / acc.x + a.dt * a.g[0], acc.z / acc.x + a.dt * a.g[1], acc.w / acc.x + a.dt * a.g[2]);
+        if (a.bc_mode) {
+          const int bk = gb % a.nb[2];
+          const int t2 = gb / a.nb[2];
+          const int bj = t2 % a.nb[1];
+          const int bi = t2 / a.nb[1];
+          const int i = bi * 4 + (lane >> 4), j = bj * 4 + ((lane >> 2) & 3), k = bk * 4 + (lane & 3);
+          if ((a.bc_mode & MPM_BC_COLLIDERS) && a.ncoll > 0)
+            v = body_collision(a.coll, a.ncoll, f3((float)i * a.h, (float)j * a.h, (float)k * a.h), v);
+          if (a.bc_mode & MPM_BC_DOMAIN) v = domain_walls(i, j, k, a.dims[0], a.dims[1], a.dims[2], v);
+        }
+        out = make_float4(v.x, v.y, v.z, acc.x);
+      }
+      if (acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f || acc.w != 0.0f)
+        a.gacc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+      a.gvel[g] = out;
+    } else {
+      const int gb = ba.glist_prev[s - ncur];
+      if (ba.gflag[gb] != ba.step) a.gvel[(int64_t)gb * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+}
+
+}  // namespace
+
+static int pass_grid(const BlockArgs& ba) {
+  const int64_t nb = (int64_t)ba.a.nb[0] * ba.a.nb[1] * ba.a.nb[2];
+  return (int)std::max<int64_t>(1, std::min<int64_t>(nb, 2048));
+}
+
+void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s) {
+  const int grid = pass_grid(ba);
+  const bool lit = ba.a.compose == COMPOSE_LITERAL;
+  switch (mode) {
+    case MODE_P2G:
+      if (lit)
+        k_block_pass<MODE_P2G, COMPOSE_LITERAL><<<grid, TPB, 0, s>>>(ba);
+      else
+        k_block_pass<MODE_P2G, COMPOSE_TEXTBOOK><<<grid, TPB, 0, s>>>(ba);
+      break;
+    case MODE_FUSED:
+      if (lit)
+        k_block_pass<MODE_FUSED, COMPOSE_LITERAL><<<grid, TPB, 0, s>>>(ba);
+      else
+        k_block_pass<MODE_FUSED, COMPOSE_TEXTBOOK><<<grid, TPB, 0, s>>>(ba);
+      break;
+    default: k_block_pass<MODE_G2P, COMPOSE_LITERAL><<<grid, TPB, 0, s>>>(ba); break;
+  }
+}
+
+void launch_grid_active(const BlockArgs& ba, hipStream_t s) {
+  const int64_t nb = (int64_t)ba.a.nb[0] * ba.a.nb[1] * ba.a.nb[2];
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((nb + 3) / 4, 2048));
+  k_grid_active<<<grid, TPB, 0, s>>>(ba);
+}
+
+}  // namespace mpm

@@ -1,0 +1,342 @@
+// Legacy-stage and layout kernels of libmpm_hip (gfx950).
+//
+// The legacy stages keep the reference's seven separate passes
+// (realtime-deformations/main.cpp:192-218), one thread per particle or per
+// node, with global float atomics into the dense block-major grid.  They back
+// the 1:1 stage calls of the C ABI and the facade; the throughput path is in
+// mpm_fused.hip.
+#include "mpm_internal.hpp"
+
+namespace mpm {
+
+namespace {
+
+__device__ __forceinline__ void load_m3(const float* P, int64_t stride, int f, int p, M3& m) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) m.c[c][r] = P[(int64_t)(f + 3 * c + r) * stride + p];
+}
+__device__ __forceinline__ void store_m3(float* P, int64_t stride, int f, int p, const M3& m) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) P[(int64_t)(f + 3 * c + r) * stride + p] = m.c[c][r];
+}
+__device__ __forceinline__ F3 load_f3(const float* P, int64_t stride, int f, int p) {
+  return f3(P[(int64_t)f * stride + p], P[(int64_t)(f + 1) * stride + p], P[(int64_t)(f + 2) * stride + p]);
+}
+__device__ __forceinline__ void store_f3(float* P, int64_t stride, int f, int p, F3 v) {
+  P[(int64_t)f * stride + p] = v.x;
+  P[(int64_t)(f + 1) * stride + p] = v.y;
+  P[(int64_t)(f + 2) * stride + p] = v.z;
+}
+
+struct Stencil {
+  Axis ax[3];
+};
+__device__ __forceinline__ Stencil stencil_of(const KArgs& a, F3 x) {
+  Stencil s;
+  s.ax[0] = axis_weights(x.x, a.h, 1, a.dims[0] - 3);
+  s.ax[1] = axis_weights(x.y, a.h, 1, a.dims[1] - 3);
+  s.ax[2] = axis_weights(x.z, a.h, 1, a.dims[2] - 3);
+  return s;
+}
+
+constexpr int TPB = 256;
+inline int nblk(int64_t n) { return (int)((n + TPB - 1) / TPB); }
+
+__global__ void k_clear(float4* p, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// rasterizeParticlesToGrid scatter: m_i += m_p w_ip,
+// p_i += w_ip m_p (v_p + B_p D^-1 (x_i - x_p))   (cpp:99-117)
+__global__ void k_stage_p2g(KArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const F3 x = load_f3(a.P, a.stride, FX, p);
+  const F3 v = load_f3(a.P, a.stride, FV, p);
+  M3 B;
+  load_m3(a.P, a.stride, FB, p, B);
+  const float m = a.P[(int64_t)FM * a.stride + p];
+  const M3 BD = mmul(B, a.Dinv);
+  const Stencil s = stencil_of(a, x);
+  for (int dx = 0; dx < 4; ++dx)
+    for (int dy = 0; dy < 4; ++dy)
+#pragma unroll
+      for (int dz = 0; dz < 4; ++dz) {
+        const int i = s.ax[0].base - 1 + dx, j = s.ax[1].base - 1 + dy, k = s.ax[2].base - 1 + dz;
+        const float w = s.ax[0].w[dx] * s.ax[1].w[dy] * s.ax[2].w[dz];
+        const F3 Xi = f3((float)i * a.h, (float)j * a.h, (float)k * a.h);
+        const F3 mom = (v + mvec(BD, Xi - x)) * (w * m);
+        float* g = reinterpret_cast<float*>(a.gacc + node_index(i, j, k, a.nb));
+        atomicAdd(g + 0, m * w);
+        atomicAdd(g + 1, mom.x);
+        atomicAdd(g + 2, mom.y);
+        atomicAdd(g + 3, mom.z);
+      }
+}
+
+__device__ __forceinline__ bool node_coords(const KArgs& a, int64_t g, int& i, int& j, int& k) {
+  const int64_t blk = g >> 6;
+  const int l = (int)(g & 63);
+  const int bk = (int)(blk % a.nb[2]);
+  const int64_t t = blk / a.nb[2];
+  const int bj = (int)(t % a.nb[1]);
+  const int bi = (int)(t / a.nb[1]);
+  i = bi * 4 + (l >> 4);
+  j = bj * 4 + ((l >> 2) & 3);
+  k = bk * 4 + (l & 3);
+  return i < a.dims[0] && j < a.dims[1] && k < a.dims[2];
+}
+
+// velocity = momentum / mass on nodes with mass (cpp:118-121)
+__global__ void k_stage_normalise(KArgs a) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nnodes) return;
+  const float4 acc = a.gacc[g];
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (acc.x != 0.0f) out = make_float4(acc.y / acc.x, acc.z / acc.x, acc.w / acc.x, acc.x);
+  a.gvel[g] = out;
+}
+
+// computeParticleVolumesAndDensities (cpp:131-142)
+__global__ void k_stage_volumes(KArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const F3 x = load_f3(a.P, a.stride, FX, p);
+  const Stencil s = stencil_of(a, x);
+  float density = 0.0f;
+  for (int dx = 0; dx < 4; ++dx)
+    for (int dy = 0; dy < 4; ++dy)
+#pragma unroll
+      for (int dz = 0; dz < 4; ++dz) {
+        const int i = s.ax[0].base - 1 + dx, j = s.ax[1].base - 1 + dy, k = s.ax[2].base - 1 + dz;
+        const float w = s.ax[0].w[dx] * s.ax[1].w[dy] * s.ax[2].w[dz];
+        density += a.gvel[node_index(i, j, k, a.nb)].w * w;
+      }
+  density /= (a.h * a.h * a.h);
+  const float m = a.P[(int64_t)FM * a.stride + p];
+  a.P[(int64_t)FVOL * a.stride + p] = density != 0.0f ? (m / density) : 0.0f;
+}
+
+// computeExplicitGridForces (cpp:235-254)
+__global__ void k_stage_forces(KArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const F3 x = load_f3(a.P, a.stride, FX, p);
+  M3 FE, FP;
+  load_m3(a.P, a.stride, FFE, p, FE);
+  load_m3(a.P, a.stride, FFP, p, FP);
+  const float V0 = a.P[(int64_t)FVOL * a.stride + p];
+  const float E = a.P[(int64_t)FE0 * a.stride + p];
+  const M3 A = stress_matrix(E, a.nu, a.xi, V0, a.Dinv, FE, FP);
+  const Stencil s = stencil_of(a, x);
+  for (int dx = 0; dx < 4; ++dx)
+    for (int dy = 0; dy < 4; ++dy)
+#pragma unroll
+      for (int dz = 0; dz < 4; ++dz) {
+        const int i = s.ax[0].base - 1 + dx, j = s.ax[1].base - 1 + dy, k = s.ax[2].base - 1 + dz;
+        const float w = s.ax[0].w[dx] * s.ax[1].w[dy] * s.ax[2].w[dz];
+        const F3 Xi = f3((float)i * a.h, (float)j * a.h, (float)k * a.h);
+        const F3 f = mvec(mscale(A, w), Xi - x);
+        float* g = reinterpret_cast<float*>(a.gforce + node_index(i, j, k, a.nb));
+        atomicAdd(g + 0, -f.x);
+        atomicAdd(g + 1, -f.y);
+        atomicAdd(g + 2, -f.z);
+      }
+}
+
+// gridVelocitiesUpdate (cpp:256-262)
+__global__ void k_stage_grid(KArgs a) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nnodes) return;
+  float4 v = a.gvel[g];
+  if (v.w == 0.0f) return;
+  const float4 f = a.gforce[g];
+  const F3 acc = f3(f.x / v.w + a.g[0], f.y / v.w + a.g[1], f.z / v.w + a.g[2]);
+  v.x = v.x + a.dt * acc.x;
+  v.y = v.y + a.dt * acc.y;
+  v.z = v.z + a.dt * acc.z;
+  a.gvel[g] = v;
+}
+
+__device__ __forceinline__ F3 apply_bc(const KArgs& a, int i, int j, int k, F3 v) {
+  if ((a.bc_mode & MPM_BC_COLLIDERS) && a.ncoll > 0)
+    v = body_collision(a.coll, a.ncoll, f3((float)i * a.h, (float)j * a.h, (float)k * a.h), v);
+  if (a.bc_mode & MPM_BC_DOMAIN) v = domain_walls(i, j, k, a.dims[0], a.dims[1], a.dims[2], v);
+  return v;
+}
+
+// gridBasedCollisions (cpp:298-304)
+__global__ void k_stage_collide(KArgs a) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nnodes) return;
+  float4 v = a.gvel[g];
+  if (v.w == 0.0f) return;
+  int i, j, k;
+  if (!node_coords(a, g, i, j, k)) return;
+  const F3 r = apply_bc(a, i, j, k, f3(v.x, v.y, v.z));
+  a.gvel[g] = make_float4(r.x, r.y, r.z, v.w);
+}
+
+// updateDeformationGradient (cpp:306-330)
+__global__ void k_stage_fupdate(KArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  M3 B, FE, FP;
+  load_m3(a.P, a.stride, FB, p, B);
+  load_m3(a.P, a.stride, FFE, p, FE);
+  load_m3(a.P, a.stride, FFP, p, FP);
+  if (!f_update(a.compose, a.dt, a.Dinv, a.lo, a.hi, a.gap64, B, FE, FP)) {
+    atomicOr(a.status, ST_NONFINITE);
+    return;
+  }
+  store_m3(a.P, a.stride, FFE, p, FE);
+  store_m3(a.P, a.stride, FFP, p, FP);
+}
+
+// updateParticleVelocities (cpp:332-342)
+__global__ void k_stage_g2p(KArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const F3 x = load_f3(a.P, a.stride, FX, p);
+  const Stencil s = stencil_of(a, x);
+  F3 v = f3(0.f, 0.f, 0.f);
+  M3 B = mdiag(0.0f);
+  for (int dx = 0; dx < 4; ++dx)
+    for (int dy = 0; dy < 4; ++dy)
+#pragma unroll
+      for (int dz = 0; dz < 4; ++dz) {
+        const int i = s.ax[0].base - 1 + dx, j = s.ax[1].base - 1 + dy, k = s.ax[2].base - 1 + dz;
+        const float w = s.ax[0].w[dx] * s.ax[1].w[dy] * s.ax[2].w[dz];
+        const float4 gv4 = a.gvel[node_index(i, j, k, a.nb)];
+        const F3 gv = f3(gv4.x, gv4.y, gv4.z);
+        v = v + gv * w;
+        const F3 d = f3((float)i * a.h, (float)j * a.h, (float)k * a.h) - x;
+        const float dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          B.c[c][0] = B.c[c][0] + (gv.x * dd[c]) * w;
+          B.c[c][1] = B.c[c][1] + (gv.y * dd[c]) * w;
+          B.c[c][2] = B.c[c][2] + (gv.z * dd[c]) * w;
+        }
+      }
+  store_f3(a.P, a.stride, FV, p, v);
+  store_m3(a.P, a.stride, FB, p, B);
+}
+
+// updateParticlePositions + clampPosition (cpp:344-350,381-388)
+__global__ void k_stage_advect(KArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  F3 x = load_f3(a.P, a.stride, FX, p);
+  const F3 v = load_f3(a.P, a.stride, FV, p);
+  x = x + v * a.dt;
+  const float lo = 3.0f * a.h;
+  x.x = fminf(fmaxf(x.x, lo), (float)(a.dims[0] - 3) * a.h);
+  x.y = fminf(fmaxf(x.y, lo), (float)(a.dims[1] - 3) * a.h);
+  x.z = fminf(fmaxf(x.z, lo), (float)(a.dims[2] - 3) * a.h);
+  store_f3(a.P, a.stride, FX, p, x);
+}
+
+__global__ void k_scatter_field(float* dst, int64_t stride, const float* src, const int* pid, int n, int comps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int q = pid ? pid[p] : p;
+  for (int c = 0; c < comps; ++c) dst[(int64_t)c * stride + p] = src[(int64_t)q * comps + c];
+}
+__global__ void k_gather_field(float* dst, const float* src, int64_t stride, const int* pid, int n, int comps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int q = pid ? pid[p] : p;
+  for (int c = 0; c < comps; ++c) dst[(int64_t)q * comps + c] = src[(int64_t)c * stride + p];
+}
+__global__ void k_iota(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+__global__ void k_fill_int(int* p, int64_t n, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_grid_to_dense(KArgs a, float* m, float* vel, float* force, uint8_t* active) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nnodes) return;
+  int i, j, k;
+  if (!node_coords(a, g, i, j, k)) return;
+  const int64_t d = ((int64_t)i * a.dims[1] + j) * a.dims[2] + k;
+  const float4 v = a.gvel[g];
+  if (m) m[d] = v.w;
+  if (vel) {
+    vel[3 * d] = v.x;
+    vel[3 * d + 1] = v.y;
+    vel[3 * d + 2] = v.z;
+  }
+  if (force) {
+    const float4 f = a.gforce[g];
+    force[3 * d] = f.x;
+    force[3 * d + 1] = f.y;
+    force[3 * d + 2] = f.z;
+  }
+  if (active) active[d] = v.w != 0.0f ? 1 : 0;
+}
+
+__global__ void k_count_active(KArgs a, int64_t* out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = g < a.nnodes && a.gvel[g].w != 0.0f;
+  const unsigned long long ball = __ballot(act);
+  if ((threadIdx.x & 63) == 0 && ball) atomicAdd(reinterpret_cast<unsigned long long*>(out), (unsigned long long)__popcll(ball));
+}
+
+}  // namespace
+
+void launch_count_active(const KArgs& a, int64_t* out, hipStream_t s) {
+  k_count_active<<<nblk(a.nnodes), TPB, 0, s>>>(a, out);
+}
+void launch_clear(float4* p, int64_t count, hipStream_t s) {
+  if (count > 0) k_clear<<<nblk(count), TPB, 0, s>>>(p, count);
+}
+void launch_stage_p2g(const KArgs& a, hipStream_t s) {
+  if (a.n) k_stage_p2g<<<nblk(a.n), TPB, 0, s>>>(a);
+}
+void launch_stage_normalise(const KArgs& a, hipStream_t s) { k_stage_normalise<<<nblk(a.nnodes), TPB, 0, s>>>(a); }
+void launch_stage_volumes(const KArgs& a, hipStream_t s) {
+  if (a.n) k_stage_volumes<<<nblk(a.n), TPB, 0, s>>>(a);
+}
+void launch_stage_forces(const KArgs& a, hipStream_t s) {
+  if (a.n) k_stage_forces<<<nblk(a.n), TPB, 0, s>>>(a);
+}
+void launch_stage_grid(const KArgs& a, hipStream_t s) { k_stage_grid<<<nblk(a.nnodes), TPB, 0, s>>>(a); }
+void launch_stage_collide(const KArgs& a, hipStream_t s) { k_stage_collide<<<nblk(a.nnodes), TPB, 0, s>>>(a); }
+void launch_stage_fupdate(const KArgs& a, hipStream_t s) {
+  if (a.n) k_stage_fupdate<<<nblk(a.n), TPB, 0, s>>>(a);
+}
+void launch_stage_g2p(const KArgs& a, hipStream_t s) {
+  if (a.n) k_stage_g2p<<<nblk(a.n), TPB, 0, s>>>(a);
+}
+void launch_stage_advect(const KArgs& a, hipStream_t s) {
+  if (a.n) k_stage_advect<<<nblk(a.n), TPB, 0, s>>>(a);
+}
+void launch_scatter_field(float* dst, int64_t stride, const float* src, const int* pid, int n, int comps,
+                          hipStream_t s) {
+  if (n) k_scatter_field<<<nblk(n), TPB, 0, s>>>(dst, stride, src, pid, n, comps);
+}
+void launch_gather_field(float* dst, const float* src, int64_t stride, const int* pid, int n, int comps,
+                         hipStream_t s) {
+  if (n) k_gather_field<<<nblk(n), TPB, 0, s>>>(dst, src, stride, pid, n, comps);
+}
+void launch_iota(int* p, int n, hipStream_t s) {
+  if (n) k_iota<<<nblk(n), TPB, 0, s>>>(p, n);
+}
+void launch_fill_int(int* p, int64_t n, int v, hipStream_t s) {
+  if (n) k_fill_int<<<nblk(n), TPB, 0, s>>>(p, n, v);
+}
+void launch_grid_to_dense(const KArgs& a, float* m, float* vel, float* force, uint8_t* active, hipStream_t s) {
+  k_grid_to_dense<<<nblk(a.nnodes), TPB, 0, s>>>(a, m, vel, force, active);
+}
+
+}  // namespace mpm

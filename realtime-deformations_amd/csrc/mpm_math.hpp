@@ -1,0 +1,529 @@
+// Device math for the MPM snow substep on gfx950.
+//
+// Matrices are glm-style column-major (c[col][row]) and every operation keeps
+// glm 0.9.7.1's evaluation order, so that with contraction disabled (this
+// library is built with -ffp-contract=off; hot loops use explicit fmaf) the
+// deformation-gradient update is bit-identical to the reference's FP32 CPU
+// path on identical inputs:
+//   realtime-deformations/material_point_method.cpp:306-330 (F update)
+//   realtime-deformations/utils.h:15-33 (the glm<->Eigen transposes)
+//   Eigen/src/SVD/JacobiSVD.h:689-819, Eigen/src/misc/RealSvd2x2.h:21-51,
+//   Eigen/src/Jacobi/Jacobi.h:96-127,297-345 (two-sided Jacobi conventions)
+// All indices into the small register arrays are compile-time constants
+// (templated pair rotations, predicated swaps), so nothing spills to scratch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mpm {
+
+struct F3 {
+  float x, y, z;
+};
+template <typename T> struct M3T {
+  T c[3][3];
+};
+using M3 = M3T<float>;
+
+__device__ __forceinline__ F3 f3(float a, float b, float c) { return {a, b, c}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+template <typename T> __device__ __forceinline__ M3T<T> mdiag(T d) {
+  M3T<T> r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.c[i][j] = (i == j) ? d : T(0);
+  return r;
+}
+template <typename T> __device__ __forceinline__ M3T<T> madd(const M3T<T>& a, const M3T<T>& b) {
+  M3T<T> r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.c[i][j] = a.c[i][j] + b.c[i][j];
+  return r;
+}
+template <typename T> __device__ __forceinline__ M3T<T> msub(const M3T<T>& a, const M3T<T>& b) {
+  M3T<T> r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.c[i][j] = a.c[i][j] - b.c[i][j];
+  return r;
+}
+template <typename T> __device__ __forceinline__ M3T<T> mscale(const M3T<T>& a, T s) {
+  M3T<T> r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.c[i][j] = a.c[i][j] * s;
+  return r;
+}
+// glm mat3*mat3: R[j][i] = A[0][i]*B[j][0] + A[1][i]*B[j][1] + A[2][i]*B[j][2]
+template <typename T> __device__ __forceinline__ M3T<T> mmul(const M3T<T>& a, const M3T<T>& b) {
+  M3T<T> r;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r.c[j][i] = a.c[0][i] * b.c[j][0] + a.c[1][i] * b.c[j][1] + a.c[2][i] * b.c[j][2];
+  return r;
+}
+__device__ __forceinline__ F3 mvec(const M3& m, F3 v) {
+  return {m.c[0][0] * v.x + m.c[1][0] * v.y + m.c[2][0] * v.z, m.c[0][1] * v.x + m.c[1][1] * v.y + m.c[2][1] * v.z,
+          m.c[0][2] * v.x + m.c[1][2] * v.y + m.c[2][2] * v.z};
+}
+template <typename T> __device__ __forceinline__ M3T<T> mtrans(const M3T<T>& a) {
+  M3T<T> r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.c[i][j] = a.c[j][i];
+  return r;
+}
+template <typename T> __device__ __forceinline__ T mdet(const M3T<T>& m) {
+  return +m.c[0][0] * (m.c[1][1] * m.c[2][2] - m.c[2][1] * m.c[1][2]) -
+         m.c[1][0] * (m.c[0][1] * m.c[2][2] - m.c[2][1] * m.c[0][2]) +
+         m.c[2][0] * (m.c[0][1] * m.c[1][2] - m.c[1][1] * m.c[0][2]);
+}
+// glm compute_inverse (cofactors times one division)
+template <typename T> __device__ __forceinline__ M3T<T> minverse(const M3T<T>& m) {
+  const T ood = T(1) / (+m.c[0][0] * (m.c[1][1] * m.c[2][2] - m.c[2][1] * m.c[1][2]) -
+                        m.c[1][0] * (m.c[0][1] * m.c[2][2] - m.c[2][1] * m.c[0][2]) +
+                        m.c[2][0] * (m.c[0][1] * m.c[1][2] - m.c[1][1] * m.c[0][2]));
+  M3T<T> r;
+  r.c[0][0] = +(m.c[1][1] * m.c[2][2] - m.c[2][1] * m.c[1][2]) * ood;
+  r.c[1][0] = -(m.c[1][0] * m.c[2][2] - m.c[2][0] * m.c[1][2]) * ood;
+  r.c[2][0] = +(m.c[1][0] * m.c[2][1] - m.c[2][0] * m.c[1][1]) * ood;
+  r.c[0][1] = -(m.c[0][1] * m.c[2][2] - m.c[2][1] * m.c[0][2]) * ood;
+  r.c[1][1] = +(m.c[0][0] * m.c[2][2] - m.c[2][0] * m.c[0][2]) * ood;
+  r.c[2][1] = -(m.c[0][0] * m.c[2][1] - m.c[2][0] * m.c[0][1]) * ood;
+  r.c[0][2] = +(m.c[0][1] * m.c[1][2] - m.c[1][1] * m.c[0][2]) * ood;
+  r.c[1][2] = -(m.c[0][0] * m.c[1][2] - m.c[1][0] * m.c[0][2]) * ood;
+  r.c[2][2] = +(m.c[0][0] * m.c[1][1] - m.c[1][0] * m.c[0][1]) * ood;
+  return r;
+}
+
+// -------------------------------------------------------- cubic B-spline
+// N(u) of material_point_method.hpp:20-31.  The reference's double literals
+// promote the polynomial to double before the final rounding; FP64 on gfx950
+// costs ~24 DP ops per particle pass, so the weights are kept bit-exact.
+__device__ __forceinline__ float weightN(float u) {
+  const float modx = fabsf(u);
+  const float modx2 = modx * modx;
+  const float modx3 = modx * modx * modx;
+  if (modx < 1.0f) return (float)(0.5 * (double)modx3 - (double)modx2 + 2.0 / 3.0);
+  if (modx < 2.0f) {
+    const float t = 2.0f - modx;
+    return (float)((1.0 / 6.0) * (double)t * (double)t * (double)t);
+  }
+  return 0.0f;
+}
+
+// Stencil base and the four non-zero 1-D weights (nodes base-1 .. base+2) of
+// one coordinate; q = x / h is the correctly rounded FP32 quotient.
+struct Axis {
+  int base;
+  float w[4];
+};
+__device__ __forceinline__ Axis axis_weights(float x, float h, int lo, int hi) {
+  Axis a;
+  const float q = x / h;
+  int b = (int)q;
+  b = b < lo ? lo : (b > hi ? hi : b);  // keeps every access in bounds
+  a.base = b;
+#pragma unroll
+  for (int d = 0; d < 4; ++d) a.w[d] = weightN(q - (float)(b - 1 + d));
+  return a;
+}
+
+// ------------------------------------------------ Eigen-convention Jacobi
+template <typename T> struct Rot {
+  T c, s;
+};
+template <typename T> struct NumLim;
+template <> struct NumLim<float> {
+  __device__ static constexpr float min() { return 1.17549435e-38f; }
+  __device__ static constexpr float eps() { return 1.1920929e-07f; }
+};
+template <> struct NumLim<double> {
+  __device__ static constexpr double min() { return 2.2250738585072014e-308; }
+  __device__ static constexpr double eps() { return 2.220446049250313e-16; }
+};
+template <typename T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x : (x == T(0) ? T(0) : x); }
+__device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
+
+// real_2x2_jacobi_svd on the (p,q) block of W (row-major math W[r][c]).
+template <typename T, int P, int Q>
+__device__ __forceinline__ void jacobi_2x2(const T (&W)[3][3], Rot<T>& jl, Rot<T>& jr) {
+  T m00 = W[P][P], m01 = W[P][Q], m10 = W[Q][P], m11 = W[Q][Q];
+  Rot<T> r1;
+  const T t = m00 + m11;
+  const T d = m10 - m01;
+  if (tabs(d) < NumLim<T>::min()) {
+    r1.s = T(0);
+    r1.c = T(1);
+  } else {
+    const T u = t / d;
+    const T tmp = tsqrt(T(1) + u * u);
+    r1.s = T(1) / tmp;
+    r1.c = u / tmp;
+  }
+  if (!(r1.c == T(1) && r1.s == T(0))) {
+    const T a0 = m00, b0 = m10, a1 = m01, b1 = m11;
+    m00 = r1.c * a0 + r1.s * b0;
+    m10 = -r1.s * a0 + r1.c * b0;
+    m01 = r1.c * a1 + r1.s * b1;
+    m11 = -r1.s * a1 + r1.c * b1;
+  }
+  const T x = m00, y = m01, z = m11;
+  const T deno = T(2) * tabs(y);
+  if (deno < NumLim<T>::min()) {
+    jr.c = T(1);
+    jr.s = T(0);
+  } else {
+    const T tau = (x - z) / deno;
+    const T w = tsqrt(tau * tau + T(1));
+    const T tt = (tau > T(0)) ? T(1) / (tau + w) : T(1) / (tau - w);
+    const T sign_t = tt > T(0) ? T(1) : T(-1);
+    const T n = T(1) / tsqrt(tt * tt + T(1));
+    jr.s = -sign_t * (y / tabs(y)) * tabs(tt) * n;
+    jr.c = n;
+  }
+  const T c2 = jr.c, s2 = -jr.s;
+  jl.c = r1.c * c2 - r1.s * s2;
+  jl.s = r1.c * s2 + r1.s * c2;
+}
+
+template <typename T, int P, int Q>
+__device__ __forceinline__ void jacobi_pair(T (&W)[3][3], T (&U)[3][3], T (&V)[3][3], T& maxDiag, bool& finished) {
+  const T pm = T(2) * NumLim<T>::eps() * maxDiag;
+  const T threshold = (NumLim<T>::min() < pm) ? pm : NumLim<T>::min();
+  if (tabs(W[P][Q]) > threshold || tabs(W[Q][P]) > threshold) {
+    finished = false;
+    Rot<T> jl, jr;
+    jacobi_2x2<T, P, Q>(W, jl, jr);
+    if (!(jl.c == T(1) && jl.s == T(0))) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {  // rows P,Q of W
+        const T xi = W[P][k], yi = W[Q][k];
+        W[P][k] = jl.c * xi + jl.s * yi;
+        W[Q][k] = -jl.s * xi + jl.c * yi;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {  // cols P,Q of U
+        const T xi = U[k][P], yi = U[k][Q];
+        U[k][P] = jl.c * xi + jl.s * yi;
+        U[k][Q] = -jl.s * xi + jl.c * yi;
+      }
+    }
+    const T rc = jr.c, rs = -jr.s;  // right rotation uses j_right^T
+    if (!(rc == T(1) && rs == T(0))) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const T xi = W[k][P], yi = W[k][Q];
+        W[k][P] = rc * xi + rs * yi;
+        W[k][Q] = -rs * xi + rc * yi;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const T xi = V[k][P], yi = V[k][Q];
+        V[k][P] = rc * xi + rs * yi;
+        V[k][Q] = -rs * xi + rc * yi;
+      }
+    }
+    const T a = tabs(W[P][P]), b = tabs(W[Q][Q]);
+    const T mx = (a < b) ? b : a;
+    maxDiag = (maxDiag < mx) ? mx : maxDiag;
+  }
+}
+
+template <typename T> __device__ __forceinline__ void swap_cols(T (&A)[3][3], int i, int j, bool doit) {
+  // i, j are compile-time after unrolling by the callers
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const T a = A[r][i], b = A[r][j];
+    A[r][i] = doit ? b : a;
+    A[r][j] = doit ? a : b;
+  }
+}
+
+// SVD of M (row-major math) = U diag(S) V^T with Eigen's conventions.
+// Returns false for non-finite input (Eigen reports InvalidInput).
+template <typename T>
+__device__ __forceinline__ bool jacobi_svd3(const T (&M)[3][3], T (&U)[3][3], T (&S)[3], T (&V)[3][3]) {
+  T scale = T(0);
+  bool bad = false;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T a = tabs(M[r][c]);
+      bad |= !(a == a) || (a > T(1e308) && sizeof(T) == 8) || (sizeof(T) == 4 && a > T(3.4028235e38f));
+      scale = (a > scale) ? a : scale;
+    }
+  if (bad) return false;
+  if (scale == T(0)) scale = T(1);
+  T W[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      W[r][c] = M[r][c] / scale;
+      U[r][c] = V[r][c] = (r == c) ? T(1) : T(0);
+    }
+  T maxDiag = tabs(W[0][0]);
+  {
+    const T b = tabs(W[1][1]);
+    maxDiag = (maxDiag < b) ? b : maxDiag;
+    const T c = tabs(W[2][2]);
+    maxDiag = (maxDiag < c) ? c : maxDiag;
+  }
+  bool finished = false;
+  for (int sweep = 0; sweep < 48 && !finished; ++sweep) {
+    finished = true;
+    jacobi_pair<T, 1, 0>(W, U, V, maxDiag, finished);
+    jacobi_pair<T, 2, 0>(W, U, V, maxDiag, finished);
+    jacobi_pair<T, 2, 1>(W, U, V, maxDiag, finished);
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const T a = W[i][i];
+    S[i] = tabs(a);
+    if (a < T(0)) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) U[r][i] = -U[r][i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) S[i] *= scale;
+  // selection sort, descending, first maximum kept
+  {
+    int pos = 0;
+    T mx = S[0];
+    if (S[1] > mx) {
+      mx = S[1];
+      pos = 1;
+    }
+    if (S[2] > mx) {
+      mx = S[2];
+      pos = 2;
+    }
+    if (mx == T(0)) return true;
+    const bool s1 = pos == 1, s2 = pos == 2;
+    T t0 = S[0];
+    S[0] = s1 ? S[1] : (s2 ? S[2] : S[0]);
+    if (s1) S[1] = t0;
+    if (s2) S[2] = t0;
+    swap_cols(U, 0, 1, s1);
+    swap_cols(V, 0, 1, s1);
+    swap_cols(U, 0, 2, s2);
+    swap_cols(V, 0, 2, s2);
+  }
+  {
+    const T mx = (S[2] > S[1]) ? S[2] : S[1];
+    if (mx == T(0)) return true;
+    const bool s2 = S[2] > S[1];
+    if (s2) {
+      const T t = S[1];
+      S[1] = S[2];
+      S[2] = t;
+    }
+    swap_cols(U, 1, 2, s2);
+    swap_cols(V, 1, 2, s2);
+  }
+  return true;
+}
+
+// ------------------------------------------------------------- polar factor
+// Rotation factor R of F = R S by scaled Newton iteration
+// X <- (g X + X^{-T} / g) / 2, g = |det X|^{-1/3}.  Like the reference's
+// Higham–Noferini routine, the result is always a rotation (minus the
+// orthogonal factor when det F < 0).
+__device__ __forceinline__ M3 polar_R(const M3& F) {
+  M3 X = F;
+  const float d0 = mdet(F);
+  for (int it = 0; it < 12; ++it) {
+    const float d = mdet(X);
+    if (!(fabsf(d) > 0.0f)) break;
+    const M3 Yt = mtrans(minverse(X));
+    const float g = 1.0f / cbrtf(fabsf(d));
+    const float ig = 1.0f / g;
+    float diff = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const float xn = 0.5f * (g * X.c[i][j] + Yt.c[i][j] * ig);
+        diff = fmaxf(diff, fabsf(xn - X.c[i][j]));
+        X.c[i][j] = xn;
+      }
+    if (diff <= 2.0e-7f) break;
+  }
+  if (d0 < 0.0f) X = mscale(X, -1.0f);
+  return X;
+}
+
+// --------------------------------------------------- deformation update
+enum { COMPOSE_LITERAL = 0, COMPOSE_TEXTBOOK = 1 };
+
+// FP64 re-solve of the clamp for TEXTBOOK mode: the singular vectors of a
+// nearly degenerate spectrum are ill-conditioned in FP32, so when the
+// relative gap falls below `gap` the SVD and the recomposition run in FP64.
+__device__ __noinline__ void textbook_fp64(const M3& FEhat, double lo, double hi, M3& FE) {
+  double M[3][3], U[3][3], S[3], V[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) M[i][j] = (double)FEhat.c[i][j];
+  jacobi_svd3<double>(M, U, S, V);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) S[i] = S[i] < lo ? lo : (S[i] > hi ? hi : S[i]);
+  // M = FEhat^T = U S V^T  ->  FE = V S U^T ; glm FE.c[col][row] = FE(row, col)
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc += V[r][k] * S[k] * U[c][k];
+      FE.c[c][r] = (float)acc;
+    }
+}
+
+// material_point_method.cpp:306-330 for one particle; Bn is the APIC matrix
+// of the previous G2P.  Returns false when the SVD rejects its input; the
+// particle's F is then left unchanged and the caller raises a status flag.
+template <int COMPOSE>
+__device__ __forceinline__ bool f_update_t(float dt, const M3& Dinv, float lo, float hi, float gap64, const M3& Bn,
+                                           M3& FE, M3& FP) {
+  const int compose = COMPOSE;
+  const M3 FPn1 = mmul(mmul(madd(mdiag(1.0f), mscale(mmul(Bn, Dinv), dt)), FE), FP);
+  const M3 FEhat = mmul(FPn1, minverse(FP));
+  float M[3][3], U[3][3], S[3], V[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) M[i][j] = FEhat.c[i][j];
+  if (!jacobi_svd3<float>(M, U, S, V)) return false;
+  M3 FEn;
+  const bool fp64 = COMPOSE == COMPOSE_TEXTBOOK && gap64 > 0.0f &&
+                    fminf(S[0] - S[1], S[1] - S[2]) < gap64 * S[0];
+  if (COMPOSE == COMPOSE_TEXTBOOK && fp64) {
+    textbook_fp64(FEhat, (double)lo, (double)hi, FEn);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) S[i] = fminf(fmaxf(S[i], lo), hi);  // std::clamp for finite values
+    M3 Ug, Vg, Sg = mdiag(0.0f);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      Sg.c[i][i] = S[i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        Ug.c[i][j] = U[i][j];
+        Vg.c[i][j] = V[i][j];
+      }
+    }
+    FEn = (compose == COMPOSE_LITERAL) ? mmul(mmul(Ug, Sg), mtrans(Vg)) : mmul(mmul(mtrans(Vg), Sg), Ug);
+  }
+  FE = FEn;
+  FP = mmul(minverse(FEn), FPn1);
+  return true;
+}
+__device__ __forceinline__ bool f_update(int compose, float dt, const M3& Dinv, float lo, float hi, float gap64,
+                                         const M3& Bn, M3& FE, M3& FP) {
+  return compose == COMPOSE_LITERAL ? f_update_t<COMPOSE_LITERAL>(dt, Dinv, lo, hi, gap64, Bn, FE, FP)
+                                    : f_update_t<COMPOSE_TEXTBOOK>(dt, Dinv, lo, hi, gap64, Bn, FE, FP);
+}
+
+// ------------------------------------------------------------ snow stress
+// V0 * Dinv * P(F_E, J_P) * F_E^T with P = 2 mu (F - R) + lambda (J-1) J F^-T,
+// mu, lambda hardened by exp(xi (1 - det F_P))
+// (material_point_method.cpp:236-251).
+__device__ __forceinline__ M3 stress_matrix(float E, float nu, float xi, float V0, const M3& Dinv, const M3& FE,
+                                            const M3& FP) {
+  const float mu0 = E / (2.0f * (1.0f + nu));
+  const float jp = mdet(FP);
+  const float hard = expf(xi * (1.0f - jp));
+  const float mu = mu0 * hard;
+  const float lambda0 = (E * nu) / ((1.0f + nu) * (1.0f - 2.0f * nu));
+  const float lambda = lambda0 * hard;
+  const M3 R = polar_R(FE);
+  const float J = mdet(FE);
+  const M3 FT = mtrans(minverse(FE));
+  const M3 P = madd(mscale(msub(FE, R), 2.0f * mu), mscale(FT, lambda * (J - 1.0f) * J));
+  return mmul(mmul(mscale(Dinv, V0), P), mtrans(FE));
+}
+
+// ---------------------------------------------------------------- colliders
+// Oriented box with the inverse pose matrix precomputed on the host in glm's
+// operation order; SDF per material_point_method.hpp:79-86.
+struct DevCollider {
+  float inv[4][4];  // glm column-major
+  float s[3];       // half extents
+  float vel[3];
+};
+__device__ __forceinline__ float box_sdf(const DevCollider& c, F3 p) {
+  float q[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const float pr = (c.inv[0][r] * p.x + c.inv[1][r] * p.y) + (c.inv[2][r] * p.z + c.inv[3][r] * 1.0f);
+    q[r] = fabsf(pr) - c.s[r];
+  }
+  float mx = q[0];
+  mx = (mx < q[1]) ? q[1] : mx;
+  mx = (mx < q[2]) ? q[2] : mx;
+  float mx0 = (mx < 0.0f) ? 0.0f : mx;
+  const float inner = fmaxf(q[0], fmaxf(q[1], q[2]));
+  return fabsf(mx0) + fminf(inner, 0.0f);
+}
+// mathy.hpp:40-56 and material_point_method.cpp:264-296
+__device__ __forceinline__ F3 body_collision(const DevCollider* cs, int k, F3 pos, F3 vel) {
+  bool all_out = true;
+  for (int i = 0; i < k; ++i)
+    if (!(box_sdf(cs[i], pos) > 0.0f)) all_out = false;
+  if (all_out) return vel;
+  F3 out = vel;
+  const float delta = 0.001f;
+  for (int i = 0; i < k; ++i) {
+    const DevCollider& c = cs[i];
+    if (box_sdf(c, pos) > 0.0f) continue;
+    float g[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const F3 d = f3(a == 0 ? delta : 0.0f, a == 1 ? delta : 0.0f, a == 2 ? delta : 0.0f);
+      const float f1 = box_sdf(c, pos - d);
+      const float f2 = box_sdf(c, pos + d);
+      g[a] = (f2 - f1) / (2.0f * delta);
+    }
+    const F3 n = f3(g[0], g[1], g[2]);
+    const F3 vobj = f3(c.vel[0], c.vel[1], c.vel[2]);
+    const F3 rel = out - vobj;
+    const float vn = dot(rel, n);
+    if (vn >= 0.0f) continue;
+    const F3 vt = rel - n * vn;
+    const float mu = 0.5f;
+    F3 vrel = f3(0.0f, 0.0f, 0.0f);
+    if (3.0f > -mu * vn) vrel = vt + vt * (mu * vn / 3.0f);  // glm vec3::length() == 3
+    out = vrel + vobj;
+  }
+  return out;
+}
+// Configs' domain walls: floor band sticky, other bands separating.
+__device__ __forceinline__ F3 domain_walls(int i, int j, int k, int I, int J, int K, F3 v) {
+  const int b = 3;
+  if (j < b) return f3(0.0f, 0.0f, 0.0f);
+  if (i < b && v.x < 0.0f) v.x = 0.0f;
+  if (i >= I - b && v.x > 0.0f) v.x = 0.0f;
+  if (j >= J - b && v.y > 0.0f) v.y = 0.0f;
+  if (k < b && v.z < 0.0f) v.z = 0.0f;
+  if (k >= K - b && v.z > 0.0f) v.z = 0.0f;
+  return v;
+}
+
+}  // namespace mpm

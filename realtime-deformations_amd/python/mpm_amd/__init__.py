@@ -1,0 +1,226 @@
+"""Python view of libmpm_hip.so through its C ABI (include/mpm_c.h).
+
+Used by the tests, the benchmark and the autograd wrapper.  It loads the
+in-tree library and fails loudly when it is missing: there is no CPU
+fallback.  Matrices are 9 floats per particle in glm column-major order
+(element (row r, col c) at index 3c + r), as in the reference's glm::mat3.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libmpm_hip.so")
+
+MPM_OK = 0
+ERRORS = {1: "MPM_ERR_ARG", 2: "MPM_ERR_HIP", 3: "MPM_ERR_BOUNDS", 4: "MPM_ERR_NONFINITE", 5: "MPM_ERR_STATE",
+          6: "MPM_ERR_NOMEM"}
+COMPOSE_LITERAL, COMPOSE_TEXTBOOK = 0, 1
+BC_COLLIDERS, BC_DOMAIN = 1, 2
+STAGE_P2G, STAGE_VOLUMES, STAGE_FORCES, STAGE_GRID, STAGE_COLLIDE, STAGE_FUPDATE, STAGE_G2P, STAGE_ADVECT = range(8)
+
+
+class MpmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dims", C.c_int32 * 3),
+        ("compose", C.c_int32),
+        ("bc_mode", C.c_int32),
+        ("flags", C.c_int32),
+        ("h", C.c_float),
+        ("E", C.c_float),
+        ("nu", C.c_float),
+        ("xi", C.c_float),
+        ("theta_c", C.c_double),
+        ("theta_s", C.c_double),
+        ("gravity", C.c_float * 3),
+        ("svd_gap_fp64", C.c_float),
+        ("resort_every", C.c_int32),
+    ]
+
+
+class Collider(C.Structure):
+    _fields_ = [
+        ("translation", C.c_float * 3),
+        ("rotation", C.c_float * 4),
+        ("scale", C.c_float * 3),
+        ("velocity", C.c_float * 3),
+    ]
+
+
+_F = C.POINTER(C.c_float)
+_lib = None
+
+# Every symbol include/mpm_c.h declares (checked by the CPU tests).
+EXPORTS = [
+    "mpm_default_params", "mpm_create", "mpm_destroy", "mpm_last_error", "mpm_num_particles",
+    "mpm_set_particles", "mpm_set_state", "mpm_set_colliders", "mpm_init_volumes", "mpm_stage",
+    "mpm_substeps", "mpm_sync", "mpm_get_particles", "mpm_get_grid", "mpm_profile_begin", "mpm_profile_end",
+    "mpm_count_active", "mpm_debug_bin",
+]
+KERNELS = {"p2g": 0, "grid": 1, "g2p": 2, "bin": 3}
+NKERNELS = 8
+
+
+def lib():
+    """Load libmpm_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with realtime-deformations_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.mpm_default_params.argtypes = [C.POINTER(Params), C.c_int32, C.c_int32, C.c_int32, C.c_float]
+        L.mpm_create.argtypes = [C.POINTER(Params), C.c_int32, C.POINTER(vp)]
+        L.mpm_destroy.argtypes = [vp]
+        L.mpm_last_error.argtypes = [vp]
+        L.mpm_last_error.restype = C.c_char_p
+        L.mpm_num_particles.argtypes = [vp]
+        L.mpm_set_particles.argtypes = [vp, _F, _F, _F, _F]
+        L.mpm_set_state.argtypes = [vp, _F, _F, _F, _F]
+        L.mpm_set_colliders.argtypes = [vp, C.POINTER(Collider), C.c_int32]
+        L.mpm_init_volumes.argtypes = [vp]
+        L.mpm_stage.argtypes = [vp, C.c_int32, C.c_float]
+        L.mpm_substeps.argtypes = [vp, C.c_float, C.c_int32]
+        L.mpm_sync.argtypes = [vp]
+        L.mpm_get_particles.argtypes = [vp] + [_F] * 6
+        L.mpm_get_grid.argtypes = [vp, _F, _F, _F, C.POINTER(C.c_uint8)]
+        L.mpm_profile_begin.argtypes = [vp]
+        L.mpm_profile_end.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+        L.mpm_count_active.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.mpm_debug_bin.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(_F)
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def default_params(dims, h, **kw) -> Params:
+    p = Params()
+    lib().mpm_default_params(C.byref(p), int(dims[0]), int(dims[1]), int(dims[2]), float(h))
+    for k, v in kw.items():
+        if k == "gravity":
+            p.gravity[:] = list(v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def collider(translation, rotation=(1.0, 0.0, 0.0, 0.0), scale=(1.0, 1.0, 1.0), velocity=(0.0, 0.0, 0.0)):
+    c = Collider()
+    c.translation[:] = list(translation)
+    c.rotation[:] = list(rotation)
+    c.scale[:] = list(scale)
+    c.velocity[:] = list(velocity)
+    return c
+
+
+class Simulation:
+    """One device context (mpm_ctx)."""
+
+    def __init__(self, params: Params, n: int):
+        self.L = lib()
+        self.n = int(n)
+        self.params = params
+        self.dims = tuple(params.dims)
+        h = C.c_void_p()
+        self._check(self.L.mpm_create(C.byref(params), self.n, C.byref(h)), h)
+        self.ctx = h
+
+    def _check(self, rc, h=None):
+        if rc != MPM_OK:
+            ctx = h if h is not None else self.ctx
+            msg = self.L.mpm_last_error(ctx).decode() if ctx is not None and ctx.value else ""
+            raise MpmError(rc, msg)
+
+    def close(self):
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            self.L.mpm_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_particles(self, x, v, m, E0=None):
+        x, v, m, E0 = _f32(x), _f32(v), _f32(m), _f32(E0)
+        self._check(self.L.mpm_set_particles(self.ctx, _fp(x), _fp(v), _fp(m), _fp(E0)))
+
+    def set_state(self, B=None, FE=None, FP=None, V0=None):
+        B, FE, FP, V0 = _f32(B), _f32(FE), _f32(FP), _f32(V0)
+        self._check(self.L.mpm_set_state(self.ctx, _fp(B), _fp(FE), _fp(FP), _fp(V0)))
+
+    def set_colliders(self, colliders):
+        arr = (Collider * max(1, len(colliders)))(*colliders)
+        self._check(self.L.mpm_set_colliders(self.ctx, arr, len(colliders)))
+
+    def init_volumes(self):
+        self._check(self.L.mpm_init_volumes(self.ctx))
+
+    def stage(self, st, dt=0.0):
+        self._check(self.L.mpm_stage(self.ctx, int(st), float(dt)))
+
+    def substeps(self, dt, n):
+        self._check(self.L.mpm_substeps(self.ctx, float(dt), int(n)))
+
+    def sync(self):
+        self._check(self.L.mpm_sync(self.ctx))
+
+    def profile_begin(self):
+        self._check(self.L.mpm_profile_begin(self.ctx))
+
+    def profile_end(self):
+        """-> (total_ms, {kernel: (ms_sum, launches)})"""
+        ms = (C.c_double * NKERNELS)()
+        cnt = (C.c_int32 * NKERNELS)()
+        tot = C.c_double()
+        self._check(self.L.mpm_profile_end(self.ctx, ms, cnt, C.byref(tot)))
+        return tot.value, {k: (ms[i], cnt[i]) for k, i in KERNELS.items()}
+
+    def count_active(self):
+        v = C.c_int64()
+        self._check(self.L.mpm_count_active(self.ctx, C.byref(v)))
+        return v.value
+
+    def debug_bin(self):
+        """-> (sorted keys, stable permutation, caller index per slot after reorder)"""
+        n = self.n
+        k, p, s = np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros(n, np.int32)
+        self._check(self.L.mpm_debug_bin(self.ctx, k.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         p.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         s.ctypes.data_as(C.POINTER(C.c_int32))))
+        return k, p, s
+
+    def particles(self):
+        n = self.n
+        out = dict(x=np.zeros((n, 3), np.float32), v=np.zeros((n, 3), np.float32), B=np.zeros((n, 9), np.float32),
+                   FE=np.zeros((n, 9), np.float32), FP=np.zeros((n, 9), np.float32), V0=np.zeros(n, np.float32))
+        self._check(self.L.mpm_get_particles(self.ctx, *[_fp(out[k]) for k in ("x", "v", "B", "FE", "FP", "V0")]))
+        return out
+
+    def grid(self, force=False):
+        cells = int(np.prod(self.dims))
+        m, vel = np.zeros(cells, np.float32), np.zeros((cells, 3), np.float32)
+        f = np.zeros((cells, 3), np.float32) if force else None
+        act = np.zeros(cells, np.uint8)
+        self._check(self.L.mpm_get_grid(self.ctx, _fp(m), _fp(vel), _fp(f), act.ctypes.data_as(C.POINTER(C.c_uint8))))
+        out = dict(m=m, vel=vel, active=act.astype(bool))
+        if force:
+            out["force"] = f
+        return out

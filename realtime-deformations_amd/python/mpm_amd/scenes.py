@@ -1,0 +1,137 @@
+"""Seeded scene generators for the snow workloads.
+
+Counter-based RNG (numpy Philox) with recorded seeds, never rand(): the
+reference's initializer draws from the platform's rand()
+(realtime-deformations/material_point_method.cpp:18-63,
+realtime-deformations/utils.h:110-127), so its particle placement cannot be
+reproduced across platforms; these generators replace it.
+
+Each generator returns a dict: x (n,3), v (n,3), m (n,), E0 (n,) float32,
+params (mpm_amd.Params keyword arguments), dims, h, dt, steps.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+RHO = 400.0  # kg/m^3
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.Philox(int(seed)))
+
+
+def _ball(rng, n, centre, radius):
+    """n points uniform in a ball (rejection sampling)."""
+    out = np.empty((0, 3))
+    while out.shape[0] < n:
+        k = int((n - out.shape[0]) * 2.2) + 64
+        p = rng.uniform(-1.0, 1.0, size=(k, 3))
+        p = p[(p * p).sum(1) <= 1.0]
+        out = np.concatenate([out, p])
+    return (np.asarray(centre) + radius * out[:n]).astype(np.float32)
+
+
+def material(theta_s=7.5e-3, E=1.4e5, bc_mode=2):
+    return dict(E=E, nu=0.2, xi=10.0, theta_c=2.5e-2, theta_s=theta_s, bc_mode=bc_mode)
+
+
+def snowball(seed=0, n=8192, grid=64, radius=0.08, centre=(0.5, 0.6, 0.5), v0=(0.0, -1.0, 0.0), E=1.4e5,
+             dt=1e-4, steps=100):
+    """Plain snowball: n particles uniform in a sphere, sticky floor and
+    separating walls.  Defaults: the 8,192-particle 64^3 free-fall case."""
+    rng = _rng(seed)
+    x = _ball(rng, n, centre, radius)
+    mp = RHO * (4.0 / 3.0) * math.pi * radius ** 3 / n
+    return dict(x=x, v=np.tile(np.float32(v0), (n, 1)), m=np.full(n, mp, np.float32),
+                E0=np.full(n, E, np.float32), dims=(grid, grid, grid), h=1.0 / grid, dt=dt, steps=steps,
+                params=material(E=E))
+
+
+def two_snowballs(seed=1, n_total=262144, grid=128, radius=0.1, dt=5e-5, steps=1000):
+    """Two colliding snowballs: centres (0.3,0.5,0.5) and (0.7,0.5,0.5),
+    velocities (+3,0,0) and (-3,0,0); N/2 particles uniform in each sphere."""
+    rng = _rng(seed)
+    half = n_total // 2
+    xa = _ball(rng, half, (0.3, 0.5, 0.5), radius)
+    xb = _ball(rng, n_total - half, (0.7, 0.5, 0.5), radius)
+    x = np.concatenate([xa, xb])
+    v = np.concatenate([np.tile(np.float32([3, 0, 0]), (half, 1)), np.tile(np.float32([-3, 0, 0]), (n_total - half, 1))])
+    mp = RHO * (4.0 / 3.0) * math.pi * radius ** 3 / half
+    return dict(x=x, v=v, m=np.full(n_total, mp, np.float32), E0=np.full(n_total, 1.4e5, np.float32),
+                dims=(grid, grid, grid), h=1.0 / grid, dt=dt, steps=steps, params=material())
+
+
+def slab(seed=2, n_total=4194304, grid=256, cells=(128, 32, 128), bottom=0.5, ppc=(4, 12), dt=2.5e-5, steps=200):
+    """Snow slab at scale: a cells[0] x cells[1] x cells[2] block of cells
+    (0.5 x 0.125 x 0.5 at dx = 1/256) with its bottom at y = `bottom`; each
+    cell's particle count drawn uniformly from ppc, then adjusted by seeded
+    +-1 steps (within ppc) so the total is exactly n_total; positions uniform
+    in their cell; E0 uniform in [1e5, 2e5]; uniform mass rho h^3 / 8."""
+    rng = _rng(seed)
+    h = 1.0 / grid
+    nc = int(np.prod(cells))
+    counts = rng.integers(ppc[0], ppc[1] + 1, size=nc)
+    diff = int(counts.sum()) - n_total
+    while diff != 0:
+        idx = rng.integers(0, nc, size=abs(diff) * 2 + 16)
+        for i in idx:
+            if diff > 0 and counts[i] > ppc[0]:
+                counts[i] -= 1
+                diff -= 1
+            elif diff < 0 and counts[i] < ppc[1]:
+                counts[i] += 1
+                diff += 1
+            if diff == 0:
+                break
+    origin_cell = np.array([(grid - cells[0]) // 2, int(round(bottom / h)), (grid - cells[2]) // 2])
+    cell_ids = np.repeat(np.arange(nc), counts)
+    ci = np.stack(np.unravel_index(cell_ids, cells), axis=1) + origin_cell
+    x = ((ci + rng.uniform(0.0, 1.0, size=(n_total, 3))) * h).astype(np.float32)
+    E0 = rng.uniform(1e5, 2e5, size=n_total).astype(np.float32)
+    return dict(x=x, v=np.zeros((n_total, 3), np.float32), m=np.full(n_total, RHO * h ** 3 / 8, np.float32), E0=E0,
+                dims=(grid, grid, grid), h=h, dt=dt, steps=steps, params=material())
+
+
+def reference_scene(seed=0, n=2147, grid=20, h=0.05, origin=(0.5, 0.6, 0.5), velocity=(0.0, -200.0, 0.0),
+                    width=0.2, mass=6e-5):
+    """The reference demo's placement (material_point_method.cpp:18-63):
+    eight quarter-offset candidates per cell of the bounding cube, each
+    jittered uniformly in a ball of radius h/4, kept if within `width` of the
+    origin; filled from the back of the particle array.  Reference constants
+    (theta_s = 5e-3, no domain walls), grid 20^3 at h = 0.05."""
+    rng = _rng(seed)
+    oc = (np.asarray(origin, np.float32) / np.float32(h)).astype(np.int32)
+    wc = int(np.float32(width) / np.float32(h))
+    ds = np.array([[1, 1, 1], [1, 1, 3], [1, 3, 1], [1, 3, 3], [3, 1, 1], [3, 1, 3], [3, 3, 1], [3, 3, 3]], np.float32)
+    pts = []
+    for i in range(oc[0] - wc, oc[0] + wc):
+        for j in range(oc[1] - wc, oc[1] + wc):
+            for k in range(oc[2] - wc, oc[2] + wc):
+                for d in ds:
+                    u = rng.uniform(-1, 1, 3)
+                    while (u * u).sum() > 1:
+                        u = rng.uniform(-1, 1, 3)
+                    p = (np.float32([i, j, k]) + d * np.float32(0.25) + np.float32(0.25) * u.astype(np.float32)) * np.float32(h)
+                    if np.linalg.norm(p - np.asarray(origin, np.float32)) > width:
+                        continue
+                    pts.append(p.astype(np.float32))
+    pts = np.array(pts[:n], np.float32)
+    n = pts.shape[0]
+    x = pts[::-1].copy()
+    return dict(x=x, v=np.tile(np.float32(velocity), (n, 1)), m=np.full(n, mass, np.float32),
+                E0=np.full(n, 1.4e5, np.float32), dims=(grid, grid, grid), h=h, dt=1e-5, steps=100,
+                params=dict(E=1.4e5, nu=0.2, xi=10.0, theta_c=2.5e-2, theta_s=5e-3, bc_mode=0))
+
+
+def reference_colliders():
+    """main.cpp's three static box colliders (realtime-deformations/main.cpp:119-156):
+    floor box T(0.5,-0.2,0.5) S(0.4); two 45-degree boxes at x = 0 and x = 1."""
+    s45, c45 = math.sin(math.radians(45.0) / 2), math.cos(math.radians(45.0) / 2)
+    q45 = (c45, 0.0, 0.0, s45)
+    return [
+        dict(translation=(0.5, -0.2, 0.5), rotation=(1.0, 0.0, 0.0, 0.0), scale=(0.4, 0.4, 0.4)),
+        dict(translation=(0.0, 0.3, 0.5), rotation=q45, scale=(0.2, 0.2, 0.3)),
+        dict(translation=(1.0, 0.3, 0.5), rotation=q45, scale=(0.2, 0.2, 0.3)),
+    ]
