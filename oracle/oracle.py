@@ -210,6 +210,38 @@ def weight(x, double=False):
     return lib().orc_weight_d(x) if double else lib().orc_weight_f(x)
 
 
+def mat3_ops(A, B):
+    """glm A*B, inverse(A), determinant(A) (column-major 9-vectors)."""
+    A, B = f32(A), f32(B)
+    p, q, d = np.zeros(9, np.float32), np.zeros(9, np.float32), np.zeros(1, np.float32)
+    lib().orc_mat3_ops_f(fptr(A), fptr(B), fptr(p), fptr(q), fptr(d))
+    return p, q, d[0]
+
+
+def sdf(c13, pos):
+    return lib().orc_sdf_f(fptr(f32(c13)), fptr(f32(pos)))
+
+
+def collider_inverse(c13):
+    out = np.zeros(16, np.float32)
+    lib().orc_collider_inverse_f(fptr(f32(c13)), fptr(out))
+    return out
+
+
+def collide(c13s, pos, vel):
+    c = f32(np.asarray(c13s).reshape(-1, 13))
+    out = np.zeros(3, np.float32)
+    lib().orc_collide_f(fptr(c), c.shape[0], fptr(f32(pos)), fptr(f32(vel)), fptr(out))
+    return out
+
+
+def domain_walls(ijk, dims, vel):
+    out = np.zeros(3, np.float32)
+    lib().orc_domain_walls_f(int(ijk[0]), int(ijk[1]), int(ijk[2]), int(dims[0]), int(dims[1]), int(dims[2]),
+                             fptr(f32(vel)), fptr(out))
+    return out
+
+
 # --------------------------------------------------------------- binning
 # No reference counterpart (the reference never sorts particles): these pin
 # the HIP binning to a CPU stable sort and to the documented round-robin
