@@ -38,15 +38,24 @@ def alg_bytes_per_particle(kernel: str, a_over_n: float) -> float:
     grid update, v (12 B) written by the grid update and read by G2P, per
     active node (A of them for N particles).
     """
-    if kernel == "p2g":  # read x v B F_E F_P m V0 E0, write F_E F_P, write m+p
+    if kernel == "fused":
+        # one particle-substep: x, v, B, F_E, F_P read and written once each,
+        # m, V0, E0 read (v and B stay in registers between the substeps it
+        # fuses, but the metric counts them), plus P2G's m+p writes and
+        # G2P's v reads on the active nodes
+        return 2 * (12 + 12 + 36 + 36 + 36) + 12 + (16 + 12) * a_over_n
+    if kernel == "p2g":  # first substep: read x v B F_E F_P m V0 E0, write F_E F_P, write m+p
         return (12 + 12 + 36 * 3 + 12) + 72 + 16 * a_over_n
-    if kernel == "g2p":  # read x and grid v, write x v B
+    if kernel == "g2p":  # last substep: read x and grid v, write x v B
         return 12 + 12 * a_over_n + (12 + 12 + 36)
     if kernel == "grid":  # read m+p, write v
         return (16 + 12) * a_over_n
-    if kernel == "bin":  # read x, write keys + index
-        return 12 + 8
+    if kernel == "bin":  # keys, 3 radix passes of (key, index), 36 fields + caller index moved
+        return 12 + 8 + 3 * (4 + 8 + 8) + 8 + 2 * (NFIELDS_BYTES + 4)
     raise KeyError(kernel)
+
+
+NFIELDS_BYTES = 36 * 4
 
 
 def substep_alg_bytes(a_over_n: float) -> float:
@@ -79,6 +88,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (calibrates rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); "
@@ -140,9 +151,27 @@ def main():
                              "frac": sub_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS},
         "kernels_ms": {k: (v[0] / max(1, v[1])) for k, v in kern.items() if v[1]},
     }
+    traffic = _profiled_traffic(dom)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic
+    if args.copy_probe:
+        sim.copy_probe(1 << 30)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, V0, args.cpu_threads)
     print(json.dumps(out), flush=True)
+
+
+def _profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel of known
+    size), or None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    entry = data.get("kernels", {}).get(kernel)
+    return None if entry is None else entry.get("bytes_per_launch")
 
 
 if __name__ == "__main__":

@@ -137,10 +137,12 @@ int mpm_get_grid(mpm_ctx* ctx, float* m, float* vel, float* force, uint8_t* acti
  * with millisecond std::chrono stamps, realtime-deformations/utils.h:138-154,
  * main.cpp:192-225) ----------------------------------------------------- */
 enum {
-  MPM_KERNEL_P2G = 0,  /* fused stress + F update + P2G scatter            */
-  MPM_KERNEL_GRID = 1, /* grid velocity update + boundary conditions        */
-  MPM_KERNEL_G2P = 2,  /* G2P + advection                                   */
-  MPM_KERNEL_BIN = 3,  /* binning: keys, radix sort, reorder                */
+  MPM_KERNEL_P2G = 0,   /* first substep of a call: stress + F update + P2G   */
+  MPM_KERNEL_GRID = 1,  /* grid velocity update + boundary conditions         */
+  MPM_KERNEL_G2P = 2,   /* end of a call: G2P + advection                     */
+  MPM_KERNEL_BIN = 3,   /* binning: keys, radix sort, reorder                 */
+  MPM_KERNEL_FUSED = 4, /* other substeps: G2P + advection + stress + F
+                           update + P2G in one pass                           */
   MPM_NKERNELS = 8
 };
 /* While profiling, every fused-path launch is bracketed by HIP events on the
@@ -151,6 +153,10 @@ int mpm_profile_begin(mpm_ctx* ctx);
 int mpm_profile_end(mpm_ctx* ctx, double* kernel_ms, int32_t* launches, double* total_ms);
 /* Number of grid nodes with non-zero mass after the last grid update. */
 int mpm_count_active(mpm_ctx* ctx, int64_t* active_nodes);
+/* A plain streaming copy of `bytes` bytes (a multiple of 16) between two
+ * device buffers on the context stream: the known-size kernel that scales
+ * the profiler's HBM byte counters. */
+int mpm_copy_probe(mpm_ctx* ctx, int64_t bytes);
 
 /* Binning check (no reference counterpart): bins the current particles as
  * the fused path does and returns, for n particles in their current device

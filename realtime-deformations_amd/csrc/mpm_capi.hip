@@ -517,7 +517,8 @@ int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps) {
       if ((rc = rebin(ctx))) return rc;
     ++ctx->step;
     const BlockArgs ba = block_args(ctx, dt);
-    if ((rc = timed(ctx, MPM_KERNEL_P2G, [&] { launch_block_pass(ba, i == 0 ? PASS_P2G : PASS_FUSED, s); })) ||
+    if ((rc = timed(ctx, i == 0 ? MPM_KERNEL_P2G : MPM_KERNEL_FUSED,
+                    [&] { launch_block_pass(ba, i == 0 ? PASS_P2G : PASS_FUSED, s); })) ||
         (rc = timed(ctx, MPM_KERNEL_GRID, [&] { launch_grid_active(ba, s); })))
       return rc;
     ++ctx->since_sort;
@@ -626,5 +627,28 @@ extern "C" int mpm_debug_bin(mpm_ctx* ctx, uint32_t* sorted_keys, uint32_t* perm
   if (slot_caller_index)
     MPM_HIP(hipMemcpyAsync(slot_caller_index, ctx->pid, sizeof(int) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
   MPM_HIP(hipStreamSynchronize(ctx->stream));
+  return MPM_OK;
+}
+
+namespace {
+__global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                    int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int mpm_copy_probe(mpm_ctx* ctx, int64_t bytes) {
+  if (check_ctx(ctx) || bytes <= 0 || bytes % 16) return MPM_ERR_ARG;
+  float4 *a = nullptr, *b = nullptr;
+  MPM_HIP(hipMalloc(&a, bytes));
+  MPM_HIP(hipMalloc(&b, bytes));
+  MPM_HIP(hipMemsetAsync(a, 0, bytes, ctx->stream));
+  const int64_t n = bytes / 16;
+  k_copy_probe<<<2048, 256, 0, ctx->stream>>>(a, b, n);
+  MPM_HIP(hipGetLastError());
+  MPM_HIP(hipStreamSynchronize(ctx->stream));
+  MPM_HIP(hipFree(a));
+  MPM_HIP(hipFree(b));
   return MPM_OK;
 }

@@ -64,9 +64,9 @@ EXPORTS = [
     "mpm_default_params", "mpm_create", "mpm_destroy", "mpm_last_error", "mpm_num_particles",
     "mpm_set_particles", "mpm_set_state", "mpm_set_colliders", "mpm_init_volumes", "mpm_stage",
     "mpm_substeps", "mpm_sync", "mpm_get_particles", "mpm_get_grid", "mpm_profile_begin", "mpm_profile_end",
-    "mpm_count_active", "mpm_debug_bin",
+    "mpm_count_active", "mpm_debug_bin", "mpm_copy_probe",
 ]
-KERNELS = {"p2g": 0, "grid": 1, "g2p": 2, "bin": 3}
+KERNELS = {"p2g": 0, "grid": 1, "g2p": 2, "bin": 3, "fused": 4}
 NKERNELS = 8
 
 
@@ -97,6 +97,7 @@ def lib():
         L.mpm_profile_end.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.mpm_count_active.argtypes = [vp, C.POINTER(C.c_int64)]
         L.mpm_debug_bin.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]
+        L.mpm_copy_probe.argtypes = [vp, C.c_int64]
         _lib = L
     return _lib
 
@@ -192,6 +193,9 @@ class Simulation:
         tot = C.c_double()
         self._check(self.L.mpm_profile_end(self.ctx, ms, cnt, C.byref(tot)))
         return tot.value, {k: (ms[i], cnt[i]) for k, i in KERNELS.items()}
+
+    def copy_probe(self, nbytes):
+        self._check(self.L.mpm_copy_probe(self.ctx, int(nbytes)))
 
     def count_active(self):
         v = C.c_int64()
