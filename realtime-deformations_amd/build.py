@@ -29,9 +29,8 @@ SOURCES = [
     "mpm_bin.hip",
     "mpm_capi.hip",
 ]
-HOST_SOURCES = [
-    "material_point_method.cpp",
-]
+FACADE = os.path.join(PKG, "facade")
+DEMO = os.path.join(LIBDIR, "facade_demo")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -63,13 +62,6 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
         obj = os.path.join(OBJDIR, src + ".o")
         cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    for src in HOST_SOURCES:
-        path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
-        obj = os.path.join(OBJDIR, src + ".o")
-        cmds.append([HIPCC, *HIP_FLAGS, "-x", "c++", "-c", path, "-o", obj])
-        objs.append(obj)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         outs = list(ex.map(_run, cmds))
     if verbose:
@@ -78,6 +70,12 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # the drop-in C++ API compiled with a caller (the reference demo's loop)
+    tmp = DEMO + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(FACADE, "material_point_method.cpp"), os.path.join(FACADE, "demo_main.cpp"),
+          "-L" + LIBDIR, "-lmpm_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp])
+    os.replace(tmp, DEMO)
     return LIB
 
 
