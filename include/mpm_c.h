@@ -126,6 +126,8 @@ int mpm_stage(mpm_ctx* ctx, int32_t stage, float dt);
  * raised by the device are reported by the next synchronising call. */
 int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps);
 int mpm_sync(mpm_ctx* ctx);
+/* Replay runs of substeps through captured HIP graphs (default 1). */
+int mpm_use_graphs(mpm_ctx* ctx, int32_t enable);
 
 /* Readback into host arrays (NULL skips a field).  getParticles()
  * (material_point_method.hpp:181-183). */
@@ -145,11 +147,12 @@ enum {
                            update + P2G in one pass                           */
   MPM_NKERNELS = 8
 };
-/* While profiling, every fused-path launch is bracketed by HIP events on the
- * context stream; mpm_profile_end synchronises and returns, per MPM_KERNEL_*
- * id, the summed kernel milliseconds and launch counts, and the wall time of
- * the whole profiled region (first to last event). */
-int mpm_profile_begin(mpm_ctx* ctx);
+/* Starts a profiled region: a HIP event on the context stream marks its
+ * start.  With per_kernel != 0 every fused-path launch is also bracketed by
+ * events (and launched directly, not through the substep HIP graph);
+ * mpm_profile_end synchronises and returns, per MPM_KERNEL_* id, the summed
+ * kernel milliseconds and launch counts, and the region's wall time. */
+int mpm_profile_begin(mpm_ctx* ctx, int32_t per_kernel);
 int mpm_profile_end(mpm_ctx* ctx, double* kernel_ms, int32_t* launches, double* total_ms);
 /* Number of grid nodes with non-zero mass after the last grid update. */
 int mpm_count_active(mpm_ctx* ctx, int64_t* active_nodes);

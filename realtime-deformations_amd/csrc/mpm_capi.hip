@@ -79,7 +79,7 @@ int check_ctx(mpm_ctx* ctx) { return ctx ? MPM_OK : MPM_ERR_ARG; }
 // Run one launch; while profiling, bracket it with a pair of HIP events on
 // the context stream.
 template <typename F> int timed(mpm_ctx* ctx, int kernel, F&& launch) {
-  if (!ctx->profiling) {
+  if (!ctx->profiling || !ctx->per_kernel) {
     launch();
     return MPM_OK;
   }
@@ -283,6 +283,7 @@ int mpm_create(const mpm_params* p, int32_t n, mpm_ctx** out) {
   alloc(&ctx->glist[0], sizeof(int) * nblocks);
   alloc(&ctx->glist[1], sizeof(int) * nblocks);
   alloc(&ctx->gcnt, sizeof(int) * 4);
+  alloc(&ctx->dstep, sizeof(int));
   alloc(&ctx->gacc, gbytes);
   alloc(&ctx->gvel, gbytes);
   alloc(&ctx->gforce, gbytes);
@@ -296,6 +297,7 @@ int mpm_create(const mpm_params* p, int32_t n, mpm_ctx** out) {
   launch_iota(ctx->pid, n, ctx->stream);
   launch_fill_int(ctx->gflag, nblocks, 0, ctx->stream);
   launch_fill_int(ctx->gcnt, 4, 0, ctx->stream);
+  launch_fill_int(ctx->dstep, 1, 1, ctx->stream);
   launch_fill_int(ctx->nblk, 1, 0, ctx->stream);
   launch_clear(ctx->gacc, ctx->nnodes, ctx->stream);
   launch_clear(ctx->gvel, ctx->nnodes, ctx->stream);
@@ -303,18 +305,23 @@ int mpm_create(const mpm_params* p, int32_t n, mpm_ctx** out) {
   MPM_HIP(hipMemsetAsync(ctx->status, 0, sizeof(int), ctx->stream));
   MPM_HIP(hipStreamSynchronize(ctx->stream));
   ctx->gacc_clean = true;
+  ctx->P_first = ctx->P;
   return MPM_OK;
 }
 
 int mpm_destroy(mpm_ctx* ctx) {
   if (!ctx) return MPM_ERR_ARG;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (hipGraphExec_t& g : ctx->gexec)
+    if (g) (void)hipGraphExecDestroy(g);
   void* bufs[] = {ctx->P,       ctx->P2,      ctx->pid,      ctx->pid2,  ctx->keys,     ctx->vals,     ctx->keys2,
                   ctx->vals2,   ctx->hist,    ctx->blk_start, ctx->blk_end, ctx->blk_ids, ctx->blist,   ctx->nblk,
-                  ctx->gflag,   ctx->glist[0], ctx->glist[1], ctx->gcnt,  ctx->gacc,     ctx->gvel,     ctx->gforce,
-                  ctx->coll,    ctx->status,  ctx->dcount};
+                  ctx->gflag,   ctx->glist[0], ctx->glist[1], ctx->gcnt,  ctx->dstep,    ctx->gacc,     ctx->gvel,
+                  ctx->gforce,  ctx->coll,    ctx->status,  ctx->dcount};
   for (void* b : bufs) (void)hipFree(b);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->ev_region)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MPM_OK;
@@ -444,12 +451,10 @@ BlockArgs block_args(mpm_ctx* ctx, float dt) {
   ba.blist = ctx->blist;
   ba.nblk = ctx->nblk;
   ba.gflag = ctx->gflag;
-  ba.step = ctx->step;
-  ba.glist_cur = ctx->glist[ctx->step & 1];
-  ba.glist_prev = ctx->glist[(ctx->step + 1) & 1];
-  ba.gcnt_cur = ctx->gcnt + ctx->step % 3;
-  ba.gcnt_prev = ctx->gcnt + (ctx->step + 2) % 3;
-  ba.gcnt_next = ctx->gcnt + (ctx->step + 1) % 3;
+  ba.glist[0] = ctx->glist[0];
+  ba.glist[1] = ctx->glist[1];
+  ba.gcnt = ctx->gcnt;
+  ba.dstep = ctx->dstep;
   return ba;
 }
 
@@ -491,6 +496,70 @@ int rebin(mpm_ctx* ctx) {
   return MPM_OK;
 }
 
+// One substep: re-bin when due, then the block pass (PASS_P2G on the first
+// substep of a call, PASS_FUSED after it), the grid pass and the epoch advance.
+int substep(mpm_ctx* ctx, float dt, bool first) {
+  int rc;
+  if (ctx->since_sort < 0 || ctx->since_sort >= ctx->resort_every)
+    if ((rc = rebin(ctx))) return rc;
+  const BlockArgs ba = block_args(ctx, dt);
+  hipStream_t s = ctx->stream;
+  if ((rc = timed(ctx, first ? MPM_KERNEL_P2G : MPM_KERNEL_FUSED,
+                  [&] { launch_block_pass(ba, first ? PASS_P2G : PASS_FUSED, s); })) ||
+      (rc = timed(ctx, MPM_KERNEL_GRID, [&] { launch_grid_active(ba, s); })))
+    return rc;
+  launch_advance_step(ctx->dstep, ctx->gcnt, s);
+  ++ctx->since_sort;
+  return MPM_OK;
+}
+
+// resort_every fused substeps that start with a re-bin, replayed from a HIP
+// graph.  The launches bake in the particle buffers (which alternate with
+// every re-bin, hence one graph per buffer parity), dt and the collider
+// table; the substep epoch and block lists are device state, so one graph
+// serves every segment.
+int graph_segment(mpm_ctx* ctx, float dt) {
+  const int par = ctx->P == ctx->P_first ? 0 : 1;
+  hipGraphExec_t& g = ctx->gexec[par];
+  if (g && (ctx->graph_dt[par] != dt || ctx->graph_coll[par] != (const void*)ctx->coll ||
+            ctx->graph_ncoll[par] != ctx->ncoll)) {
+    MPM_HIP(hipGraphExecDestroy(g));
+    g = nullptr;
+  }
+  if (!g) {
+    float *const P0 = ctx->P, *const P20 = ctx->P2;
+    int *const pid0 = ctx->pid, *const pid20 = ctx->pid2;
+    const int since0 = ctx->since_sort;
+    hipGraph_t graph = nullptr;
+    MPM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    int rc = MPM_OK;
+    for (int i = 0; i < ctx->resort_every && !rc; ++i) rc = substep(ctx, dt, false);
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+    // capture executed nothing: restore the host view of the state
+    ctx->P = P0;
+    ctx->P2 = P20;
+    ctx->pid = pid0;
+    ctx->pid2 = pid20;
+    ctx->since_sort = since0;
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    MPM_HIP(ce);
+    const hipError_t ie = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MPM_HIP(ie);
+    ctx->graph_dt[par] = dt;
+    ctx->graph_coll[par] = ctx->coll;
+    ctx->graph_ncoll[par] = ctx->ncoll;
+  }
+  MPM_HIP(hipGraphLaunch(g, ctx->stream));
+  std::swap(ctx->P, ctx->P2);
+  std::swap(ctx->pid, ctx->pid2);
+  ctx->since_sort = ctx->resort_every;
+  return MPM_OK;
+}
+
 }  // namespace
 
 int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps) {
@@ -508,20 +577,19 @@ int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps) {
     launch_clear(ctx->gvel, ctx->nnodes, s);
     launch_fill_int(ctx->gflag, nblocks, 0, s);
     launch_fill_int(ctx->gcnt, 4, 0, s);
-    ctx->step = 0;
+    launch_fill_int(ctx->dstep, 1, 1, s);
     ctx->gacc_clean = true;
   }
+  const bool graphs = ctx->use_graphs && !(ctx->profiling && ctx->per_kernel);
   int rc;
-  for (int i = 0; i < nsteps; ++i) {
-    if (ctx->since_sort < 0 || ctx->since_sort >= ctx->resort_every)
-      if ((rc = rebin(ctx))) return rc;
-    ++ctx->step;
-    const BlockArgs ba = block_args(ctx, dt);
-    if ((rc = timed(ctx, i == 0 ? MPM_KERNEL_P2G : MPM_KERNEL_FUSED,
-                    [&] { launch_block_pass(ba, i == 0 ? PASS_P2G : PASS_FUSED, s); })) ||
-        (rc = timed(ctx, MPM_KERNEL_GRID, [&] { launch_grid_active(ba, s); })))
-      return rc;
-    ++ctx->since_sort;
+  for (int i = 0; i < nsteps;) {
+    if (graphs && i > 0 && ctx->since_sort >= ctx->resort_every && nsteps - i >= ctx->resort_every) {
+      if ((rc = graph_segment(ctx, dt))) return rc;
+      i += ctx->resort_every;
+    } else {
+      if ((rc = substep(ctx, dt, i == 0))) return rc;
+      ++i;
+    }
   }
   const BlockArgs ba = block_args(ctx, dt);
   if ((rc = timed(ctx, MPM_KERNEL_G2P, [&] { launch_block_pass(ba, PASS_G2P, s); }))) return rc;
@@ -529,17 +597,32 @@ int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps) {
   return MPM_OK;
 }
 
-int mpm_profile_begin(mpm_ctx* ctx) {
+int mpm_use_graphs(mpm_ctx* ctx, int32_t enable) {
+  if (check_ctx(ctx)) return MPM_ERR_ARG;
+  ctx->use_graphs = enable != 0;
+  return MPM_OK;
+}
+
+int mpm_profile_begin(mpm_ctx* ctx, int32_t per_kernel) {
   if (check_ctx(ctx)) return MPM_ERR_ARG;
   MPM_HIP(hipStreamSynchronize(ctx->stream));
+  for (hipEvent_t& e : ctx->ev_region)
+    if (!e) MPM_HIP(hipEventCreate(&e));
   ctx->profiling = true;
+  ctx->per_kernel = per_kernel != 0;
   ctx->events_used = 0;
   ctx->event_kernel.clear();
+  MPM_HIP(hipEventRecord(ctx->ev_region[0], ctx->stream));
   return MPM_OK;
 }
 
 int mpm_profile_end(mpm_ctx* ctx, double* kernel_ms, int32_t* launches, double* total_ms) {
   if (check_ctx(ctx)) return MPM_ERR_ARG;
+  if (!ctx->profiling) {
+    ctx->err = "mpm_profile_end without mpm_profile_begin";
+    return MPM_ERR_STATE;
+  }
+  MPM_HIP(hipEventRecord(ctx->ev_region[1], ctx->stream));
   MPM_HIP(hipStreamSynchronize(ctx->stream));
   ctx->profiling = false;
   for (int k = 0; k < MPM_NKERNELS; ++k) {
@@ -554,12 +637,9 @@ int mpm_profile_end(mpm_ctx* ctx, double* kernel_ms, int32_t* launches, double* 
     if (launches) launches[k] += 1;
   }
   if (total_ms) {
-    *total_ms = 0.0;
-    if (ctx->events_used >= 2) {
-      float ms = 0.0f;
-      MPM_HIP(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[ctx->events_used - 1]));
-      *total_ms = ms;
-    }
+    float ms = 0.0f;
+    MPM_HIP(hipEventElapsedTime(&ms, ctx->ev_region[0], ctx->ev_region[1]));
+    *total_ms = ms;
   }
   return MPM_OK;
 }

@@ -40,9 +40,10 @@ __device__ __forceinline__ void mark_block(const BlockArgs& ba, int bi, int bj, 
   const KArgs& a = ba.a;
   if (bi < 0 || bj < 0 || bk < 0 || bi >= a.nb[0] || bj >= a.nb[1] || bk >= a.nb[2]) return;
   const int gb = (bi * a.nb[1] + bj) * a.nb[2] + bk;
-  if (atomicExch(&ba.gflag[gb], ba.step) != ba.step) {
-    const int slot = atomicAdd(ba.gcnt_cur, 1);
-    ba.glist_cur[slot] = gb;
+  const int step = *ba.dstep;
+  if (atomicExch(&ba.gflag[gb], step) != step) {
+    const int slot = atomicAdd(ba.gcnt + step % 3, 1);
+    ba.glist[step & 1][slot] = gb;
   }
 }
 
@@ -347,12 +348,14 @@ __global__ void __launch_bounds__(TPB) k_grid_active(BlockArgs ba) {
   const KArgs& a = ba.a;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *ba.gcnt_next = 0;
-  const int ncur = *ba.gcnt_cur;
-  const int nprev = *ba.gcnt_prev;
+  const int step = *ba.dstep;
+  const int* glist_cur = ba.glist[step & 1];
+  const int* glist_prev = ba.glist[(step + 1) & 1];
+  const int ncur = ba.gcnt[step % 3];
+  const int nprev = ba.gcnt[(step + 2) % 3];
   for (int s = blockIdx.x * 4 + wave; s < ncur + nprev; s += gridDim.x * 4) {
     if (s < ncur) {
-      const int gb = ba.glist_cur[s];
+      const int gb = glist_cur[s];
       const int64_t g = (int64_t)gb * 64 + lane;
       const float4 acc = a.gacc[g];
       float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -374,13 +377,23 @@ __global__ void __launch_bounds__(TPB) k_grid_active(BlockArgs ba) {
         a.gacc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
       a.gvel[g] = out;
     } else {
-      const int gb = ba.glist_prev[s - ncur];
-      if (ba.gflag[gb] != ba.step) a.gvel[(int64_t)gb * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+      const int gb = glist_prev[s - ncur];
+      if (ba.gflag[gb] != step) a.gvel[(int64_t)gb * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
 }
 
+// Next substep: advance the epoch and clear the counter it will fill (last
+// used two substeps ago).
+__global__ void k_advance_step(int* dstep, int* gcnt) {
+  const int s = *dstep;
+  *dstep = s + 1;
+  gcnt[(s + 1) % 3] = 0;
+}
+
 }  // namespace
+
+void launch_advance_step(int* dstep, int* gcnt, hipStream_t s) { k_advance_step<<<1, 1, 0, s>>>(dstep, gcnt); }
 
 static int pass_grid(const BlockArgs& ba) {
   const int64_t nb = (int64_t)ba.a.nb[0] * ba.a.nb[1] * ba.a.nb[2];

@@ -4,21 +4,23 @@
 
 namespace mpm {
 
+// The substep epoch lives on the device (*dstep, advanced by
+// launch_advance_step after each grid pass), so one captured substep can be
+// replayed as a HIP graph: grid-block lists alternate with the epoch's
+// parity, their counters rotate with it modulo 3.
 struct BlockArgs {
   KArgs a;
   const int4* blist;  // (home block, start, end, 0)
   const int* nblk;    // number of home blocks
-  int* gflag;         // per grid block: substep epoch of its last activation
-  int* glist_cur;     // grid blocks activated this substep
-  int* gcnt_cur;
-  const int* glist_prev;  // grid blocks activated last substep
-  const int* gcnt_prev;
-  int* gcnt_next;     // reset for the next substep
-  int step;           // substep epoch (> 0)
+  int* gflag;         // per grid block: epoch of its last activation
+  int* glist[2];      // grid blocks activated, by epoch parity
+  int* gcnt;          // [3] counters, by epoch modulo 3
+  const int* dstep;   // substep epoch (> 0)
 };
 
 enum { PASS_P2G = 0, PASS_FUSED = 1, PASS_G2P = 2 };
 void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s);
 void launch_grid_active(const BlockArgs& ba, hipStream_t s);
+void launch_advance_step(int* dstep, int* gcnt, hipStream_t s);
 
 }  // namespace mpm

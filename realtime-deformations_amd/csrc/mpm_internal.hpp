@@ -79,7 +79,7 @@ struct mpm_ctx {
   int* gflag = nullptr;
   int* glist[2] = {nullptr, nullptr};
   int* gcnt = nullptr;      // three rotating counters
-  int step = 0;             // substep epoch
+  int* dstep = nullptr;     // device substep epoch (> 0)
   int since_sort = -1;      // substeps since the last binning (-1: never binned)
   int resort_every = 16;
   float4* gacc = nullptr;
@@ -92,8 +92,18 @@ struct mpm_ctx {
   bool particles_set = false;
   bool gacc_clean = false;  // fused path needs zeroed accumulators
   std::string err;
+  // HIP graphs of [rebin + resort_every substeps], one per particle-buffer
+  // parity, valid for one dt and collider table
+  float* P_first = nullptr;  // P as allocated: parity 0 when P == P_first
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  float graph_dt[2] = {0.f, 0.f};
+  const void* graph_coll[2] = {nullptr, nullptr};
+  int graph_ncoll[2] = {-1, -1};
+  bool use_graphs = true;
   // event profiling of fused-path launches
   bool profiling = false;
+  bool per_kernel = true;
+  hipEvent_t ev_region[2] = {nullptr, nullptr};  // profiled region begin, end
   std::vector<hipEvent_t> events;   // pool, reused across profiles
   size_t events_used = 0;
   std::vector<int> event_kernel;    // kernel id per (start, stop) pair

@@ -64,7 +64,7 @@ EXPORTS = [
     "mpm_default_params", "mpm_create", "mpm_destroy", "mpm_last_error", "mpm_num_particles",
     "mpm_set_particles", "mpm_set_state", "mpm_set_colliders", "mpm_init_volumes", "mpm_stage",
     "mpm_substeps", "mpm_sync", "mpm_get_particles", "mpm_get_grid", "mpm_profile_begin", "mpm_profile_end",
-    "mpm_count_active", "mpm_debug_bin", "mpm_copy_probe",
+    "mpm_count_active", "mpm_debug_bin", "mpm_copy_probe", "mpm_use_graphs",
 ]
 KERNELS = {"p2g": 0, "grid": 1, "g2p": 2, "bin": 3, "fused": 4}
 NKERNELS = 8
@@ -93,7 +93,8 @@ def lib():
         L.mpm_sync.argtypes = [vp]
         L.mpm_get_particles.argtypes = [vp] + [_F] * 6
         L.mpm_get_grid.argtypes = [vp, _F, _F, _F, C.POINTER(C.c_uint8)]
-        L.mpm_profile_begin.argtypes = [vp]
+        L.mpm_profile_begin.argtypes = [vp, C.c_int32]
+        L.mpm_use_graphs.argtypes = [vp, C.c_int32]
         L.mpm_profile_end.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.mpm_count_active.argtypes = [vp, C.POINTER(C.c_int64)]
         L.mpm_debug_bin.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]
@@ -183,8 +184,14 @@ class Simulation:
     def sync(self):
         self._check(self.L.mpm_sync(self.ctx))
 
-    def profile_begin(self):
-        self._check(self.L.mpm_profile_begin(self.ctx))
+    def use_graphs(self, enable=True):
+        """Replay runs of substeps from captured HIP graphs (the default)."""
+        self._check(self.L.mpm_use_graphs(self.ctx, int(bool(enable))))
+
+    def profile_begin(self, per_kernel=True):
+        """Start a profiled region.  per_kernel brackets every launch with HIP
+        events (and launches directly, bypassing the substep graphs)."""
+        self._check(self.L.mpm_profile_begin(self.ctx, int(bool(per_kernel))))
 
     def profile_end(self):
         """-> (total_ms, {kernel: (ms_sum, launches)})"""
