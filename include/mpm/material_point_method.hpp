@@ -23,6 +23,13 @@
 //  - Math types are glm's when <glm/glm.hpp> is on the include path (as in
 //    the reference), otherwise the small value types below with the same
 //    layout and element access (m[col][row]).
+//
+// Defining MPM_FACADE_IN_TREE builds the header inside the reference's own
+// tree, where main.cpp also includes mesh.hpp and logger.hpp: ftype, v3t,
+// m3t, Logger, Loggable, Object3D and Mesh then come from the reference's
+// constants.hpp, logger.hpp and mesh.hpp, and MeshCollider holds a drawable
+// Mesh built from the reference constructor's GL arguments (only its pose is
+// sent to the GPU).
 #pragma once
 
 #include <array>
@@ -35,7 +42,20 @@
 
 #include "../mpm_c.h"
 
-#if __has_include(<glm/glm.hpp>) && !defined(MPM_FACADE_NO_GLM)
+#if defined(MPM_FACADE_IN_TREE)
+#include <glm/glm.hpp>
+#include <constants.hpp>
+#include <logger.hpp>
+#include <mesh.hpp>
+namespace mpmf {
+using vec3 = glm::vec3;
+using ivec3 = glm::ivec3;
+using mat3 = glm::mat3;
+using mat4 = glm::mat4;
+using quat = glm::quat;
+}  // namespace mpmf
+#define MPM_FACADE_GLM 1
+#elif __has_include(<glm/glm.hpp>) && !defined(MPM_FACADE_NO_GLM)
 #include <glm/glm.hpp>
 #include <glm/gtc/quaternion.hpp>
 namespace mpmf {
@@ -92,6 +112,7 @@ struct quat {
 }  // namespace mpmf
 #endif
 
+#ifndef MPM_FACADE_IN_TREE
 typedef float ftype;
 typedef mpmf::vec3 v3t;
 typedef mpmf::mat3 m3t;
@@ -134,6 +155,7 @@ class Object3D {
   // taken apart again (translate * rotate * scale matrices, no shear).
   void applyMatrix4(const mpmf::mat4& m);
 };
+#endif  // !MPM_FACADE_IN_TREE
 
 namespace MaterialPointMethod {
 const auto DEFAULT_LOG_LEVEL_OPTIMIZER = Logger::LogLevel::WARNING;
@@ -169,6 +191,20 @@ inline v3t wipGrad(mpmf::ivec3 idx, v3t pos) {
 
 // realtime-deformations/material_point_method.hpp:74-93: an oriented box
 // collider; `mesh` holds its pose, scale is the half extent.
+#ifdef MPM_FACADE_IN_TREE
+// The reference's collider (material_point_method.hpp:74-93): a drawable
+// Mesh, its velocity and a box SDF of the mesh pose.
+class MeshCollider {
+ public:
+  Mesh mesh;
+  MeshCollider(GLuint programID, glm::mat4& VP, const std::vector<GLfloat>& vertices,
+               const std::vector<GLfloat>& colors, const glm::vec3& vel);
+  MeshCollider(const MeshCollider& o);
+  std::function<ftype(const v3t&)> sdf;
+  glm::vec3 velocity;
+  void move(ftype timeDelta);
+};
+#else
 class MeshCollider {
  public:
   Object3D mesh;
@@ -184,6 +220,7 @@ class MeshCollider {
   MeshCollider& operator=(const MeshCollider& o);
   void move(ftype timeDelta);
 };
+#endif
 
 // realtime-deformations/material_point_method.hpp:96-110
 struct Particle {

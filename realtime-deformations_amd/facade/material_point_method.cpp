@@ -100,6 +100,7 @@ v3t random_in_ball(SplitMix& g, ftype R) {
 
 }  // namespace
 
+#ifndef MPM_FACADE_IN_TREE
 void Object3D::applyMatrix4(const mpmf::mat4& m) {
   matrixWorld = mul4(m, matrixWorld);
   for (int a = 0; a < 3; ++a) translation[a] = matrixWorld[3][a];
@@ -112,6 +113,7 @@ void Object3D::applyMatrix4(const mpmf::mat4& m) {
   }
   rotation = rot_to_quat(R);
 }
+#endif
 
 namespace MaterialPointMethod {
 
@@ -136,6 +138,18 @@ ftype box_sdf(const Object3D& o, const v3t& pos) {
 }
 }  // namespace
 
+// sdf reads this collider's own pose, also in copies (the reference's lambda
+// keeps pointing at the object it was built in).
+#ifdef MPM_FACADE_IN_TREE
+MeshCollider::MeshCollider(GLuint programID, glm::mat4& VP, const std::vector<GLfloat>& vertices,
+                           const std::vector<GLfloat>& colors, const glm::vec3& vel)
+    : mesh{programID, VP, vertices, colors}, velocity(vel) {
+  sdf = [this](const v3t& p) -> ftype { return box_sdf(mesh, p); };
+}
+MeshCollider::MeshCollider(const MeshCollider& o) : mesh(o.mesh), velocity(o.velocity) {
+  sdf = [this](const v3t& p) -> ftype { return box_sdf(mesh, p); };
+}
+#else
 MeshCollider::MeshCollider(const v3t& vel) : velocity(vel) {
   sdf = [this](const v3t& p) -> ftype { return box_sdf(mesh, p); };
 }
@@ -148,6 +162,7 @@ MeshCollider& MeshCollider::operator=(const MeshCollider& o) {
   sdf = [this](const v3t& p) -> ftype { return box_sdf(mesh, p); };
   return *this;
 }
+#endif
 void MeshCollider::move(ftype timeDelta) {
   mpmf::mat4 T(1.0f);
   for (int a = 0; a < 3; ++a) T[3][a] = timeDelta * velocity[a];
