@@ -31,6 +31,7 @@ SOURCES = [
 ]
 FACADE = os.path.join(PKG, "facade")
 DEMO = os.path.join(LIBDIR, "facade_demo")
+HOSTCHECK = os.path.join(LIBDIR, "libmpm_hostcheck.so")
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -76,6 +77,11 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
           os.path.join(FACADE, "material_point_method.cpp"), os.path.join(FACADE, "demo_main.cpp"),
           "-L" + LIBDIR, "-lmpm_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp])
     os.replace(tmp, DEMO)
+    # host build of the device math, for the CPU checks against the oracle
+    tmp = HOSTCHECK + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+          "-I" + CSRC, os.path.join(CSRC, "mpm_hostcheck.cpp"), "-o", tmp])
+    os.replace(tmp, HOSTCHECK)
     return LIB
 
 

@@ -148,13 +148,19 @@ KArgs make_args(const mpm_ctx* c, float dt) {
   }
   a.nnodes = c->nnodes;
   a.h = c->h;
+  a.inv_h = 1.0f / c->h;
   a.dt = dt;
   a.lo = c->lo;
   a.hi = c->hi;
   a.gap64 = c->prm.svd_gap_fp64;
   a.Dinv = c->Dinv;
-  a.nu = c->prm.nu;
-  a.xi = c->prm.xi;
+  const float nu = c->prm.nu;
+  a.lame.nu = nu;
+  a.lame.xi = c->prm.xi;
+  a.lame.dmu = 2.0f * (1.0f + nu);
+  a.lame.rmu = 1.0f / a.lame.dmu;
+  a.lame.dlam = (1.0f + nu) * (1.0f - 2.0f * nu);
+  a.lame.rlam = 1.0f / a.lame.dlam;
   a.compose = c->prm.compose;
   a.bc_mode = c->prm.bc_mode;
   a.ncoll = c->ncoll;

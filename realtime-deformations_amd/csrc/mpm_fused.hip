@@ -33,8 +33,16 @@ constexpr int PLANE = 1008;  // >= 8*SX + 8*SY + 8 + 1
 
 enum Mode { MODE_P2G = 0, MODE_FUSED = 1, MODE_G2P = 2 };
 
-__device__ __forceinline__ float ld(const KArgs& a, int f, int p) { return a.P[(int64_t)f * a.stride + p]; }
-__device__ __forceinline__ void st(const KArgs& a, int f, int p, float v) { a.P[(int64_t)f * a.stride + p] = v; }
+// Field f of slot p: a wave-uniform row base plus a 32-bit byte offset, which
+// maps to one scalar-base global access per field (slots < 2^30).
+__device__ __forceinline__ float ld(const KArgs& a, int f, int p) {
+  const char* row = reinterpret_cast<const char*>(a.P + (int64_t)f * a.stride);
+  return *reinterpret_cast<const float*>(row + (uint32_t)p * 4u);
+}
+__device__ __forceinline__ void st(const KArgs& a, int f, int p, float v) {
+  char* row = reinterpret_cast<char*>(a.P + (int64_t)f * a.stride);
+  *reinterpret_cast<float*>(row + (uint32_t)p * 4u) = v;
+}
 
 __device__ __forceinline__ void mark_block(const BlockArgs& ba, int bi, int bj, int bk) {
   const KArgs& a = ba.a;
@@ -116,9 +124,9 @@ struct StencilInfo {
 };
 __device__ __forceinline__ StencilInfo stencil_at(const KArgs& a, const int o[3], F3 x) {
   StencilInfo s;
-  s.ax[0] = axis_weights(x.x, a.h, 1, a.dims[0] - 3);
-  s.ax[1] = axis_weights(x.y, a.h, 1, a.dims[1] - 3);
-  s.ax[2] = axis_weights(x.z, a.h, 1, a.dims[2] - 3);
+  s.ax[0] = axis_weights(x.x, a.h, a.inv_h, 1, a.dims[0] - 3);
+  s.ax[1] = axis_weights(x.y, a.h, a.inv_h, 1, a.dims[1] - 3);
+  s.ax[2] = axis_weights(x.z, a.h, a.inv_h, 1, a.dims[2] - 3);
   s.fits = true;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -202,7 +210,7 @@ __device__ __forceinline__ void p2g_one(const BlockArgs& ba, double* __restrict_
     m = ld(a, FM, p);
     const float V0 = ld(a, FVOL, p);
     const float E = ld(a, FE0, p);
-    const M3 A = stress_matrix(E, a.nu, a.xi, V0, a.Dinv, FE, FP);
+    const M3 A = stress_matrix(E, a.lame, V0, a.Dinv, FE, FP);
     const M3 BD = mmul(B, a.Dinv);
 #pragma unroll
     for (int c = 0; c < 3; ++c)

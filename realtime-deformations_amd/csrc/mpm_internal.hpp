@@ -42,9 +42,9 @@ struct KArgs {
   int dims[3];
   int nb[3];
   int64_t nnodes;  // nb0*nb1*nb2*64
-  float h, dt, lo, hi, gap64;
+  float h, inv_h, dt, lo, hi, gap64;  // inv_h: 1/h correctly rounded
   M3 Dinv;
-  float nu, xi;
+  Lame lame;
   float g[3];
   int compose, bc_mode, ncoll;
   const DevCollider* coll;

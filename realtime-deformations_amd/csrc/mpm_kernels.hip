@@ -37,9 +37,9 @@ struct Stencil {
 };
 __device__ __forceinline__ Stencil stencil_of(const KArgs& a, F3 x) {
   Stencil s;
-  s.ax[0] = axis_weights(x.x, a.h, 1, a.dims[0] - 3);
-  s.ax[1] = axis_weights(x.y, a.h, 1, a.dims[1] - 3);
-  s.ax[2] = axis_weights(x.z, a.h, 1, a.dims[2] - 3);
+  s.ax[0] = axis_weights(x.x, a.h, a.inv_h, 1, a.dims[0] - 3);
+  s.ax[1] = axis_weights(x.y, a.h, a.inv_h, 1, a.dims[1] - 3);
+  s.ax[2] = axis_weights(x.z, a.h, a.inv_h, 1, a.dims[2] - 3);
   return s;
 }
 
@@ -132,7 +132,7 @@ __global__ void k_stage_forces(KArgs a) {
   load_m3(a.P, a.stride, FFP, p, FP);
   const float V0 = a.P[(int64_t)FVOL * a.stride + p];
   const float E = a.P[(int64_t)FE0 * a.stride + p];
-  const M3 A = stress_matrix(E, a.nu, a.xi, V0, a.Dinv, FE, FP);
+  const M3 A = stress_matrix(E, a.lame, V0, a.Dinv, FE, FP);
   const Stencil s = stencil_of(a, x);
   for (int dx = 0; dx < 4; ++dx)
     for (int dy = 0; dy < 4; ++dy)

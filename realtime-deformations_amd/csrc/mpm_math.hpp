@@ -11,8 +11,22 @@
 //   Eigen/src/Jacobi/Jacobi.h:96-127,297-345 (two-sided Jacobi conventions)
 // All indices into the small register arrays are compile-time constants
 // (templated pair rotations, predicated swaps), so nothing spills to scratch.
+//
+// The header also compiles as plain host C++ (MPM_HD expands to `inline`),
+// so the same source of the weights, the divisions and the F update can be
+// checked exhaustively on the CPU against the oracle.
 #pragma once
+#if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
+#define MPM_HD __host__ __device__ __forceinline__
+#define MPM_HDC __host__ __device__
+#define MPM_HD_NOINLINE inline __host__ __device__ __noinline__
+#else
+#include <cmath>
+#define MPM_HD inline
+#define MPM_HDC
+#define MPM_HD_NOINLINE inline
+#endif
 
 #include <cstdint>
 
@@ -26,13 +40,13 @@ template <typename T> struct M3T {
 };
 using M3 = M3T<float>;
 
-__device__ __forceinline__ F3 f3(float a, float b, float c) { return {a, b, c}; }
-__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+MPM_HD F3 f3(float a, float b, float c) { return {a, b, c}; }
+MPM_HD F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+MPM_HD F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+MPM_HD F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+MPM_HD float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
-template <typename T> __device__ __forceinline__ M3T<T> mdiag(T d) {
+template <typename T> MPM_HD M3T<T> mdiag(T d) {
   M3T<T> r;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -40,7 +54,7 @@ template <typename T> __device__ __forceinline__ M3T<T> mdiag(T d) {
     for (int j = 0; j < 3; ++j) r.c[i][j] = (i == j) ? d : T(0);
   return r;
 }
-template <typename T> __device__ __forceinline__ M3T<T> madd(const M3T<T>& a, const M3T<T>& b) {
+template <typename T> MPM_HD M3T<T> madd(const M3T<T>& a, const M3T<T>& b) {
   M3T<T> r;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -48,7 +62,7 @@ template <typename T> __device__ __forceinline__ M3T<T> madd(const M3T<T>& a, co
     for (int j = 0; j < 3; ++j) r.c[i][j] = a.c[i][j] + b.c[i][j];
   return r;
 }
-template <typename T> __device__ __forceinline__ M3T<T> msub(const M3T<T>& a, const M3T<T>& b) {
+template <typename T> MPM_HD M3T<T> msub(const M3T<T>& a, const M3T<T>& b) {
   M3T<T> r;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -56,7 +70,7 @@ template <typename T> __device__ __forceinline__ M3T<T> msub(const M3T<T>& a, co
     for (int j = 0; j < 3; ++j) r.c[i][j] = a.c[i][j] - b.c[i][j];
   return r;
 }
-template <typename T> __device__ __forceinline__ M3T<T> mscale(const M3T<T>& a, T s) {
+template <typename T> MPM_HD M3T<T> mscale(const M3T<T>& a, T s) {
   M3T<T> r;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -65,7 +79,7 @@ template <typename T> __device__ __forceinline__ M3T<T> mscale(const M3T<T>& a, 
   return r;
 }
 // glm mat3*mat3: R[j][i] = A[0][i]*B[j][0] + A[1][i]*B[j][1] + A[2][i]*B[j][2]
-template <typename T> __device__ __forceinline__ M3T<T> mmul(const M3T<T>& a, const M3T<T>& b) {
+template <typename T> MPM_HD M3T<T> mmul(const M3T<T>& a, const M3T<T>& b) {
   M3T<T> r;
 #pragma unroll
   for (int j = 0; j < 3; ++j)
@@ -73,11 +87,11 @@ template <typename T> __device__ __forceinline__ M3T<T> mmul(const M3T<T>& a, co
     for (int i = 0; i < 3; ++i) r.c[j][i] = a.c[0][i] * b.c[j][0] + a.c[1][i] * b.c[j][1] + a.c[2][i] * b.c[j][2];
   return r;
 }
-__device__ __forceinline__ F3 mvec(const M3& m, F3 v) {
+MPM_HD F3 mvec(const M3& m, F3 v) {
   return {m.c[0][0] * v.x + m.c[1][0] * v.y + m.c[2][0] * v.z, m.c[0][1] * v.x + m.c[1][1] * v.y + m.c[2][1] * v.z,
           m.c[0][2] * v.x + m.c[1][2] * v.y + m.c[2][2] * v.z};
 }
-template <typename T> __device__ __forceinline__ M3T<T> mtrans(const M3T<T>& a) {
+template <typename T> MPM_HD M3T<T> mtrans(const M3T<T>& a) {
   M3T<T> r;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -85,13 +99,13 @@ template <typename T> __device__ __forceinline__ M3T<T> mtrans(const M3T<T>& a) 
     for (int j = 0; j < 3; ++j) r.c[i][j] = a.c[j][i];
   return r;
 }
-template <typename T> __device__ __forceinline__ T mdet(const M3T<T>& m) {
+template <typename T> MPM_HD T mdet(const M3T<T>& m) {
   return +m.c[0][0] * (m.c[1][1] * m.c[2][2] - m.c[2][1] * m.c[1][2]) -
          m.c[1][0] * (m.c[0][1] * m.c[2][2] - m.c[2][1] * m.c[0][2]) +
          m.c[2][0] * (m.c[0][1] * m.c[1][2] - m.c[1][1] * m.c[0][2]);
 }
 // glm compute_inverse (cofactors times one division)
-template <typename T> __device__ __forceinline__ M3T<T> minverse(const M3T<T>& m) {
+template <typename T> MPM_HD M3T<T> minverse(const M3T<T>& m) {
   const T ood = T(1) / (+m.c[0][0] * (m.c[1][1] * m.c[2][2] - m.c[2][1] * m.c[1][2]) -
                         m.c[1][0] * (m.c[0][1] * m.c[2][2] - m.c[2][1] * m.c[0][2]) +
                         m.c[2][0] * (m.c[0][1] * m.c[1][2] - m.c[1][1] * m.c[0][2]));
@@ -108,36 +122,59 @@ template <typename T> __device__ __forceinline__ M3T<T> minverse(const M3T<T>& m
   return r;
 }
 
+// ------------------------------------------------- division by a constant
+// a / b correctly rounded, given rb = 1/b correctly rounded (computed once on
+// the host): q = a*rb is within one ulp of a/b, the residual a - b*q is exact
+// under a fused multiply-add, and q + r*rb rounds to the correctly rounded
+// quotient (Markstein's correction; the quotients here are normal numbers).
+MPM_HD float div_rcp(float a, float b, float rb) {
+  const float q = a * rb;
+  const float r = fmaf(-q, b, a);
+  return fmaf(r, rb, q);
+}
+
 // -------------------------------------------------------- cubic B-spline
 // N(u) of material_point_method.hpp:20-31.  The reference's double literals
 // promote the polynomial to double before the final rounding; FP64 on gfx950
-// costs ~24 DP ops per particle pass, so the weights are kept bit-exact.
-__device__ __forceinline__ float weightN(float u) {
-  const float modx = fabsf(u);
+// runs at the FP32 rate, so the weights are kept bit-exact.
+MPM_HD float weight_inner(float modx) {  // |u| < 1
   const float modx2 = modx * modx;
   const float modx3 = modx * modx * modx;
-  if (modx < 1.0f) return (float)(0.5 * (double)modx3 - (double)modx2 + 2.0 / 3.0);
-  if (modx < 2.0f) {
-    const float t = 2.0f - modx;
-    return (float)((1.0 / 6.0) * (double)t * (double)t * (double)t);
-  }
+  return (float)(0.5 * (double)modx3 - (double)modx2 + 2.0 / 3.0);
+}
+MPM_HD float weight_outer(float modx) {  // 1 <= |u| < 2
+  const float t = 2.0f - modx;
+  return (float)((1.0 / 6.0) * (double)t * (double)t * (double)t);
+}
+MPM_HD float weightN(float u) {
+  const float modx = fabsf(u);
+  if (modx < 1.0f) return weight_inner(modx);
+  if (modx < 2.0f) return weight_outer(modx);
   return 0.0f;
 }
 
 // Stencil base and the four non-zero 1-D weights (nodes base-1 .. base+2) of
-// one coordinate; q = x / h is the correctly rounded FP32 quotient.
+// one coordinate; q = x / h is the correctly rounded FP32 quotient.  With
+// b = int(q) and q >= 2 every offset u = q - node is exact and its branch of
+// N is known: u0 in [1,2), u1 in [0,1), |u2| in (0,1], |u3| in (1,2].  The
+// two edge values |u2| = 1 and |u3| = 2 (q an integer) give the same float
+// from either branch (1/6 and 0), so the branch-free form equals weightN.
+// Positions are kept in [3h, (dims-3)h], where the clamp of b never binds.
 struct Axis {
   int base;
   float w[4];
 };
-__device__ __forceinline__ Axis axis_weights(float x, float h, int lo, int hi) {
+MPM_HD Axis axis_weights(float x, float h, float inv_h, int lo, int hi) {
   Axis a;
-  const float q = x / h;
+  const float q = div_rcp(x, h, inv_h);
   int b = (int)q;
   b = b < lo ? lo : (b > hi ? hi : b);  // keeps every access in bounds
   a.base = b;
-#pragma unroll
-  for (int d = 0; d < 4; ++d) a.w[d] = weightN(q - (float)(b - 1 + d));
+  const float fb = (float)b;
+  a.w[0] = weight_outer(q - (fb - 1.0f));
+  a.w[1] = weight_inner(q - fb);
+  a.w[2] = weight_inner(fabsf(q - (fb + 1.0f)));
+  a.w[3] = weight_outer(fabsf(q - (fb + 2.0f)));
   return a;
 }
 
@@ -147,20 +184,20 @@ template <typename T> struct Rot {
 };
 template <typename T> struct NumLim;
 template <> struct NumLim<float> {
-  __device__ static constexpr float min() { return 1.17549435e-38f; }
-  __device__ static constexpr float eps() { return 1.1920929e-07f; }
+  MPM_HDC static constexpr float min() { return 1.17549435e-38f; }
+  MPM_HDC static constexpr float eps() { return 1.1920929e-07f; }
 };
 template <> struct NumLim<double> {
-  __device__ static constexpr double min() { return 2.2250738585072014e-308; }
-  __device__ static constexpr double eps() { return 2.220446049250313e-16; }
+  MPM_HDC static constexpr double min() { return 2.2250738585072014e-308; }
+  MPM_HDC static constexpr double eps() { return 2.220446049250313e-16; }
 };
-template <typename T> __device__ __forceinline__ T tabs(T x) { return x < T(0) ? -x : (x == T(0) ? T(0) : x); }
-__device__ __forceinline__ float tsqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double tsqrt(double x) { return sqrt(x); }
+template <typename T> MPM_HD T tabs(T x) { return x < T(0) ? -x : (x == T(0) ? T(0) : x); }
+MPM_HD float tsqrt(float x) { return sqrtf(x); }
+MPM_HD double tsqrt(double x) { return sqrt(x); }
 
 // real_2x2_jacobi_svd on the (p,q) block of W (row-major math W[r][c]).
 template <typename T, int P, int Q>
-__device__ __forceinline__ void jacobi_2x2(const T (&W)[3][3], Rot<T>& jl, Rot<T>& jr) {
+MPM_HD void jacobi_2x2(const T (&W)[3][3], Rot<T>& jl, Rot<T>& jr) {
   T m00 = W[P][P], m01 = W[P][Q], m10 = W[Q][P], m11 = W[Q][Q];
   Rot<T> r1;
   const T t = m00 + m11;
@@ -192,7 +229,8 @@ __device__ __forceinline__ void jacobi_2x2(const T (&W)[3][3], Rot<T>& jl, Rot<T
     const T tt = (tau > T(0)) ? T(1) / (tau + w) : T(1) / (tau - w);
     const T sign_t = tt > T(0) ? T(1) : T(-1);
     const T n = T(1) / tsqrt(tt * tt + T(1));
-    jr.s = -sign_t * (y / tabs(y)) * tabs(tt) * n;
+    const T sgn_y = y < T(0) ? T(-1) : T(1);  // y / |y|, y != 0 here
+    jr.s = -sign_t * sgn_y * tabs(tt) * n;
     jr.c = n;
   }
   const T c2 = jr.c, s2 = -jr.s;
@@ -201,7 +239,7 @@ __device__ __forceinline__ void jacobi_2x2(const T (&W)[3][3], Rot<T>& jl, Rot<T
 }
 
 template <typename T, int P, int Q>
-__device__ __forceinline__ void jacobi_pair(T (&W)[3][3], T (&U)[3][3], T (&V)[3][3], T& maxDiag, bool& finished) {
+MPM_HD void jacobi_pair(T (&W)[3][3], T (&U)[3][3], T (&V)[3][3], T& maxDiag, bool& finished) {
   const T pm = T(2) * NumLim<T>::eps() * maxDiag;
   const T threshold = (NumLim<T>::min() < pm) ? pm : NumLim<T>::min();
   if (tabs(W[P][Q]) > threshold || tabs(W[Q][P]) > threshold) {
@@ -243,7 +281,7 @@ __device__ __forceinline__ void jacobi_pair(T (&W)[3][3], T (&U)[3][3], T (&V)[3
   }
 }
 
-template <typename T> __device__ __forceinline__ void swap_cols(T (&A)[3][3], int i, int j, bool doit) {
+template <typename T> MPM_HD void swap_cols(T (&A)[3][3], int i, int j, bool doit) {
   // i, j are compile-time after unrolling by the callers
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -256,7 +294,7 @@ template <typename T> __device__ __forceinline__ void swap_cols(T (&A)[3][3], in
 // SVD of M (row-major math) = U diag(S) V^T with Eigen's conventions.
 // Returns false for non-finite input (Eigen reports InvalidInput).
 template <typename T>
-__device__ __forceinline__ bool jacobi_svd3(const T (&M)[3][3], T (&U)[3][3], T (&S)[3], T (&V)[3][3]) {
+MPM_HD bool jacobi_svd3(const T (&M)[3][3], T (&U)[3][3], T (&S)[3], T (&V)[3][3]) {
   T scale = T(0);
   bool bad = false;
 #pragma unroll
@@ -345,7 +383,7 @@ __device__ __forceinline__ bool jacobi_svd3(const T (&M)[3][3], T (&U)[3][3], T 
 // X <- (g X + X^{-T} / g) / 2, g = |det X|^{-1/3}.  Like the reference's
 // Higham–Noferini routine, the result is always a rotation (minus the
 // orthogonal factor when det F < 0).
-__device__ __forceinline__ M3 polar_R(const M3& F) {
+MPM_HD M3 polar_R(const M3& F) {
   M3 X = F;
   const float d0 = mdet(F);
   for (int it = 0; it < 12; ++it) {
@@ -375,7 +413,7 @@ enum { COMPOSE_LITERAL = 0, COMPOSE_TEXTBOOK = 1 };
 // FP64 re-solve of the clamp for TEXTBOOK mode: the singular vectors of a
 // nearly degenerate spectrum are ill-conditioned in FP32, so when the
 // relative gap falls below `gap` the SVD and the recomposition run in FP64.
-__device__ __noinline__ void textbook_fp64(const M3& FEhat, double lo, double hi, M3& FE) {
+MPM_HD_NOINLINE void textbook_fp64(const M3& FEhat, double lo, double hi, M3& FE) {
   double M[3][3], U[3][3], S[3], V[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -400,7 +438,7 @@ __device__ __noinline__ void textbook_fp64(const M3& FEhat, double lo, double hi
 // of the previous G2P.  Returns false when the SVD rejects its input; the
 // particle's F is then left unchanged and the caller raises a status flag.
 template <int COMPOSE>
-__device__ __forceinline__ bool f_update_t(float dt, const M3& Dinv, float lo, float hi, float gap64, const M3& Bn,
+MPM_HD bool f_update_t(float dt, const M3& Dinv, float lo, float hi, float gap64, const M3& Bn,
                                            M3& FE, M3& FP) {
   const int compose = COMPOSE;
   const M3 FPn1 = mmul(mmul(madd(mdiag(1.0f), mscale(mmul(Bn, Dinv), dt)), FE), FP);
@@ -435,7 +473,7 @@ __device__ __forceinline__ bool f_update_t(float dt, const M3& Dinv, float lo, f
   FP = mmul(minverse(FEn), FPn1);
   return true;
 }
-__device__ __forceinline__ bool f_update(int compose, float dt, const M3& Dinv, float lo, float hi, float gap64,
+MPM_HD bool f_update(int compose, float dt, const M3& Dinv, float lo, float hi, float gap64,
                                          const M3& Bn, M3& FE, M3& FP) {
   return compose == COMPOSE_LITERAL ? f_update_t<COMPOSE_LITERAL>(dt, Dinv, lo, hi, gap64, Bn, FE, FP)
                                     : f_update_t<COMPOSE_TEXTBOOK>(dt, Dinv, lo, hi, gap64, Bn, FE, FP);
@@ -445,13 +483,18 @@ __device__ __forceinline__ bool f_update(int compose, float dt, const M3& Dinv, 
 // V0 * Dinv * P(F_E, J_P) * F_E^T with P = 2 mu (F - R) + lambda (J-1) J F^-T,
 // mu, lambda hardened by exp(xi (1 - det F_P))
 // (material_point_method.cpp:236-251).
-__device__ __forceinline__ M3 stress_matrix(float E, float nu, float xi, float V0, const M3& Dinv, const M3& FE,
-                                            const M3& FP) {
-  const float mu0 = E / (2.0f * (1.0f + nu));
+// Lame denominators 2(1+nu) and (1+nu)(1-2nu) with their correctly rounded
+// reciprocals, computed once on the host.
+struct Lame {
+  float nu, xi, dmu, rmu, dlam, rlam;
+};
+MPM_HD M3 stress_matrix(float E, const Lame& L, float V0, const M3& Dinv, const M3& FE, const M3& FP) {
+  const float nu = L.nu, xi = L.xi;
+  const float mu0 = div_rcp(E, L.dmu, L.rmu);  // E / (2 (1 + nu))
   const float jp = mdet(FP);
   const float hard = expf(xi * (1.0f - jp));
   const float mu = mu0 * hard;
-  const float lambda0 = (E * nu) / ((1.0f + nu) * (1.0f - 2.0f * nu));
+  const float lambda0 = div_rcp(E * nu, L.dlam, L.rlam);  // E nu / ((1 + nu)(1 - 2 nu))
   const float lambda = lambda0 * hard;
   const M3 R = polar_R(FE);
   const float J = mdet(FE);
@@ -468,7 +511,7 @@ struct DevCollider {
   float s[3];       // half extents
   float vel[3];
 };
-__device__ __forceinline__ float box_sdf(const DevCollider& c, F3 p) {
+MPM_HD float box_sdf(const DevCollider& c, F3 p) {
   float q[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -483,7 +526,7 @@ __device__ __forceinline__ float box_sdf(const DevCollider& c, F3 p) {
   return fabsf(mx0) + fminf(inner, 0.0f);
 }
 // mathy.hpp:40-56 and material_point_method.cpp:264-296
-__device__ __forceinline__ F3 body_collision(const DevCollider* cs, int k, F3 pos, F3 vel) {
+MPM_HD F3 body_collision(const DevCollider* cs, int k, F3 pos, F3 vel) {
   bool all_out = true;
   for (int i = 0; i < k; ++i)
     if (!(box_sdf(cs[i], pos) > 0.0f)) all_out = false;
@@ -515,7 +558,7 @@ __device__ __forceinline__ F3 body_collision(const DevCollider* cs, int k, F3 po
   return out;
 }
 // Configs' domain walls: floor band sticky, other bands separating.
-__device__ __forceinline__ F3 domain_walls(int i, int j, int k, int I, int J, int K, F3 v) {
+MPM_HD F3 domain_walls(int i, int j, int k, int I, int J, int K, F3 v) {
   const int b = 3;
   if (j < b) return f3(0.0f, 0.0f, 0.0f);
   if (i < b && v.x < 0.0f) v.x = 0.0f;

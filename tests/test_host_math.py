@@ -1,0 +1,125 @@
+"""The library's math source (csrc/mpm_math.hpp) built for the host and held
+to the oracle bit for bit on the CPU: B-spline weights of whole stencils,
+the constant divisions by h and by the Lame denominators, the Jacobi SVD and
+the deformation-gradient update in both compositions.
+
+The same source is compiled for gfx950 with the same flags; the GPU parity
+tests check the device build against the oracle too."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import warm_state
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "realtime-deformations_amd", "lib", "libmpm_hostcheck.so")
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int)
+
+
+@pytest.fixture(scope="module")
+def hc():
+    if not os.path.exists(LIB):
+        pytest.fail(f"{LIB} missing: run realtime-deformations_amd/build.py")
+    L = C.CDLL(LIB)
+    L.hc_axis_weights.argtypes = [_F, C.c_int, C.c_float, C.c_int, C.c_int, _I, _F]
+    L.hc_weightN.argtypes = [_F, C.c_int, _F]
+    L.hc_div_rcp.argtypes = [_F, C.c_int, C.c_float, _F]
+    L.hc_svd3.argtypes = [_F] * 4
+    L.hc_fupdate.argtypes = [C.c_int, C.c_float, C.c_float, C.c_double, C.c_double] + [_F] * 5
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(_F)
+
+
+def _axis(hc, x, h, lo, hi):
+    x = np.ascontiguousarray(x, np.float32)
+    base = np.zeros(x.size, np.int32)
+    w = np.zeros((x.size, 4), np.float32)
+    hc.hc_axis_weights(_p(x), x.size, h, lo, hi, base.ctypes.data_as(_I), _p(w))
+    return base, w
+
+
+@pytest.mark.parametrize("h,dims", [(1 / 64, 64), (1 / 256, 256), (0.05, 20), (1 / 96, 96), (0.0123, 80)])
+def test_stencil_weights_equal_reference_polynomial(hc, h, dims):
+    """base = int(x/h) with the correctly rounded quotient and the four
+    non-zero weights N(x/h - node) equal the oracle's weightN of every node,
+    for positions across the particle range, including exact node
+    positions where x/h is an integer."""
+    h = np.float32(h)
+    rng = np.random.default_rng(5)
+    lo_x, hi_x = np.float32(3) * h, np.float32(dims - 3) * h
+    x = rng.uniform(lo_x, hi_x, 20000).astype(np.float32)
+    nodes = (np.arange(3, dims - 3, dtype=np.float32) * h).astype(np.float32)
+    x = np.concatenate([x, nodes, np.nextafter(nodes, np.float32(0)), np.nextafter(nodes, np.float32(9))])
+    x = x[(x >= lo_x) & (x <= hi_x)]
+    base, w = _axis(hc, x, h, 1, dims - 3)
+    q = x / h  # numpy float32 division: correctly rounded
+    np.testing.assert_array_equal(base, q.astype(np.int32))
+    ref = np.empty_like(w)
+    for i in range(x.size):
+        for d in range(4):
+            ref[i, d] = O.weight(float(np.float32(q[i] - np.float32(base[i] - 1 + d))))
+    np.testing.assert_array_equal(w, ref)
+
+
+def test_weightN_matches_oracle(hc):
+    u = np.concatenate([np.linspace(-2.5, 2.5, 20001, dtype=np.float32),
+                        np.array([0, 1, -1, 2, -2, 0.5, 1.5], np.float32)])
+    out = np.zeros_like(u)
+    hc.hc_weightN(_p(u), u.size, _p(out))
+    np.testing.assert_array_equal(out, np.array([O.weight(float(v)) for v in u], np.float32))
+
+
+@pytest.mark.parametrize("b", [0.05, 1 / 3, 1 / 96, 1 / 100, 0.0123, 2.4, 0.72, 7.0, 1e-3, 1.7e4])
+def test_constant_division_is_correctly_rounded(hc, b):
+    """div_rcp(a, b, 1/b) == a / b (IEEE) for 2^21 dividend significands
+    spread over the binade, at several exponents: the quotient's rounding
+    depends on the significands only, so this covers the division by h and
+    by 2(1+nu), (1+nu)(1-2nu) over the ranges the kernels use."""
+    b = np.float32(b)
+    # every fourth significand of [1, 2), then scaled by powers of two
+    a1 = (np.arange(0, 1 << 23, 4, dtype=np.uint32) | np.uint32(127 << 23)).view(np.float32)
+    for e in (-10, -3, 0, 4, 17):
+        a = (a1 * np.float32(2.0 ** e)).astype(np.float32)
+        out = np.zeros_like(a)
+        hc.hc_div_rcp(_p(a), a.size, b, _p(out))
+        ref = a / b
+        bad = np.flatnonzero(out != ref)
+        assert bad.size == 0, f"b={b} e={e}: {bad.size} misrounded, first a={a[bad[0]]!r}"
+
+
+def test_jacobi_svd_bit_exact_against_oracle(hc):
+    rng = np.random.default_rng(17)
+    cases = [np.eye(3) + rng.normal(0, s, (3, 3)) for s in (1e-6, 1e-4, 1e-2, 1e-1) for _ in range(200)]
+    cases += [rng.normal(size=(3, 3)) for _ in range(300)]
+    cases += [np.diag([1.0, 1.0, 0.5]), np.diag([2.0, 2.0, 2.0]), np.zeros((3, 3)), np.diag([1.0, -1.0, 1.0])]
+    for M in cases:
+        M = np.ascontiguousarray(M, np.float32).ravel()
+        U, S, V = np.zeros(9, np.float32), np.zeros(3, np.float32), np.zeros(9, np.float32)
+        rc = hc.hc_svd3(_p(M), _p(U), _p(S), _p(V))
+        orc, Uo, So, Vo = O.svd3(M.reshape(3, 3))
+        assert rc == 0 and orc == 0
+        np.testing.assert_array_equal(U, Uo.ravel())
+        np.testing.assert_array_equal(S, So)
+        np.testing.assert_array_equal(V, Vo.ravel())
+
+
+@pytest.mark.parametrize("compose", [0, 1])
+def test_f_update_bit_exact_against_oracle(hc, compose):
+    rng = np.random.default_rng(23 + compose)
+    n = 3000
+    for bscale in (0.05, 3.0):
+        B, FE, FP = warm_state(rng, n, bscale=bscale)
+        for p in range(n):
+            FEo, FPo = np.zeros(9, np.float32), np.zeros(9, np.float32)
+            rc = hc.hc_fupdate(compose, 1e-4, 1 / 64, 2.5e-2, 7.5e-3, _p(B[p]), _p(FE[p]), _p(FP[p]), _p(FEo), _p(FPo))
+            orc, FEr, FPr, _ = O.fupdate(compose, 1e-4, 1 / 64, 2.5e-2, 7.5e-3, B[p], FE[p], FP[p])
+            assert rc == 0 and orc == 0
+            np.testing.assert_array_equal(FEo, FEr)
+            np.testing.assert_array_equal(FPo, FPr)
