@@ -207,10 +207,38 @@ __global__ void __launch_bounds__(256) k_block_detect(BinArgs b) {
   if (i == b.n - 1 || (b.keys[i + 1] >> 6) != blk) b.blk_end[blk] = i + 1;
 }
 
-// Move every listed block's particles to their round-robin slots.
+// Position of the k-th (0-based) set bit of m.
+__device__ __forceinline__ int select_bit(unsigned long long m, int k) {
+  int pos = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const unsigned long long low = m & ((1ull << w) - 1ull);
+    const int c = __popcll(low);
+    if (k >= c) {
+      k -= c;
+      m >>= w;
+      pos += w;
+    } else {
+      m = low;
+    }
+  }
+  return pos;
+}
+
+// Move every listed block's particles to their round-robin slots: slot
+// start + R[r] + k holds the r-th particle of the k-th cell that has more than
+// r particles, R[r] = sum_c' min(cnt[c'], r) being the particles of earlier
+// rounds.  One wave tabulates, per round, the ballot of cells that still have
+// particles and R; every thread then walks destination slots (coalesced
+// writes) and finds its source with one bit select.  Blocks with a cell of
+// more than RR_MAX particles take the per-source formula instead.
+constexpr int RR_MAX = 64;
 __global__ void __launch_bounds__(256) k_reorder(BinArgs b) {
   __shared__ int cnt[64];
   __shared__ int cstart[64];
+  __shared__ unsigned long long rmask[RR_MAX];
+  __shared__ int rbase[RR_MAX + 1];
+  __shared__ int nround;
   const int t = threadIdx.x;
   for (int s = blockIdx.x; s < *b.nblk; s += gridDim.x) {
     const int blk = b.blk_ids[s];
@@ -227,20 +255,48 @@ __global__ void __launch_bounds__(256) k_reorder(BinArgs b) {
         if (t >= off) incl += v;
       }
       cstart[t] = incl - c;
+      int base = 0, r = 0;
+      for (; r < RR_MAX; ++r) {
+        const unsigned long long m = __ballot(c > r);
+        if (m == 0ull) break;
+        if (t == 0) {
+          rmask[r] = m;
+          rbase[r] = base;
+        }
+        base += __popcll(m);
+      }
+      const bool more = __ballot(c > RR_MAX) != 0ull;
+      if (t == 0) {
+        rbase[r] = base;
+        nround = more ? -1 : r;
+      }
     }
     __syncthreads();
-    for (int i = start + t; i < end; i += 256) {
-      const int c = (int)(b.keys[i] & 63u);
-      const int r = i - start - cstart[c];
-      int pos = 0;
-      for (int cc = 0; cc < 64; ++cc) {
-        const int k = cnt[cc];
-        pos += (k < r ? k : r) + ((cc < c && k > r) ? 1 : 0);
+    const int nr = nround;
+    if (nr >= 0) {
+      for (int dst = start + t; dst < end; dst += 256) {
+        const int j = dst - start;
+        int r = 0;
+        while (r + 1 < nr && rbase[r + 1] <= j) ++r;
+        const int c = select_bit(rmask[r], j - rbase[r]);
+        const int src = (int)b.vals[start + cstart[c] + r];
+        for (int f = 0; f < NFIELDS; ++f) b.Pout[(int64_t)f * b.stride + dst] = b.P[(int64_t)f * b.stride + src];
+        b.pid_out[dst] = b.pid[src];
       }
-      const int dst = start + pos;
-      const int src = (int)b.vals[i];
-      for (int f = 0; f < NFIELDS; ++f) b.Pout[(int64_t)f * b.stride + dst] = b.P[(int64_t)f * b.stride + src];
-      b.pid_out[dst] = b.pid[src];
+    } else {
+      for (int i = start + t; i < end; i += 256) {
+        const int c = (int)(b.keys[i] & 63u);
+        const int r = i - start - cstart[c];
+        int pos = 0;
+        for (int cc = 0; cc < 64; ++cc) {
+          const int k = cnt[cc];
+          pos += (k < r ? k : r) + ((cc < c && k > r) ? 1 : 0);
+        }
+        const int dst = start + pos;
+        const int src = (int)b.vals[i];
+        for (int f = 0; f < NFIELDS; ++f) b.Pout[(int64_t)f * b.stride + dst] = b.P[(int64_t)f * b.stride + src];
+        b.pid_out[dst] = b.pid[src];
+      }
     }
     if (t == 0) b.blist[s] = make_int4(blk, start, end, 0);
     __syncthreads();
