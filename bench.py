@@ -93,11 +93,17 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device-to-device copy (calibrates rocprof byte counters)")
+    ap.add_argument("--workload", default="slab4m", choices=["slab4m", "batch64"],
+                    help="slab4m: the headline 4M-particle slab; batch64: 64 independent scenes in one "
+                         "context against the same scenes run one after another")
+    ap.add_argument("--autotune", action="store_true", help="batch64: autotune the batch before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); "
               "multi-GPU sharding of the substep is a design note, not a measured path.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch64":
+        return bench_batch(args)
 
     import numpy as np
     import mpm_amd as M
@@ -172,6 +178,83 @@ def main():
         sim.copy_probe(1 << 30)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, V0, args.cpu_threads)
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """64 scenes (N_s log-uniform in [4096, 262144], 64^3 or 128^3 grids,
+    E0 log-uniform in [5e4, 5e5], v0 uniform in [-3, 3]^3, dt 1e-4): one
+    batched context against one context per scene run one after another."""
+    import numpy as np
+    import mpm_amd as M
+    from mpm_amd import scenes
+
+    scs = scenes.scene_batch()
+    ns = [s["x"].shape[0] for s in scs]
+    dt = scs[0]["dt"]
+    k = args.steps
+
+    def params(sc):
+        prm = sc["params"]
+        return M.default_params(sc["dims"], sc["h"], bc_mode=prm["bc_mode"], E=prm["E"], nu=prm["nu"], xi=prm["xi"],
+                                theta_c=prm["theta_c"], theta_s=prm["theta_s"])
+
+    b = M.Simulation([params(s) for s in scs], ns)
+    for i, s in enumerate(scs):
+        b.scene_set_particles(i, s["x"], s["v"], s["m"], s["E0"])
+    b.init_volumes()
+    tuned = None
+    if args.autotune:
+        tuned = dict(ms_per_substep=b.autotune(dt, 16))
+        tuned["resort_every"], tuned["block_grid"] = b.tuning()
+    b.substeps(dt, args.warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.substeps(dt, k)
+    b.sync()
+    batch_s = time.perf_counter() - t0
+    status = b.scene_status()
+    b.close()
+
+    singles = []
+    for s in scs:
+        g = M.Simulation(params(s), s["x"].shape[0])
+        g.set_particles(s["x"], s["v"], s["m"], s["E0"])
+        g.init_volumes()
+        g.substeps(dt, args.warmup)
+        g.sync()
+        singles.append(g)
+    t0 = time.perf_counter()
+    for g in singles:
+        g.substeps(dt, k)
+        g.sync()
+    seq_s = time.perf_counter() - t0
+    for g in singles:
+        g.close()
+
+    total = int(sum(ns))
+    value = total * k / batch_s
+    out = {
+        "metric": "batched particle-substeps/s, 64 scenes (speed-up vs one scene at a time)",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": k,
+        "warmup": args.warmup,
+        "ms_per_step": batch_s / k * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "64 snowball scenes, N_s log-U[4096, 262144], 64^3/128^3, dt 1e-4",
+                   "particles": total, "scenes": len(scs), "grids_128": int(sum(1 for s in scs if s["dims"][0] == 128))},
+        "sequential": {"value": total * k / seq_s, "ms_per_step": seq_s / k * 1e3},
+        "speedup_vs_sequential": seq_s / batch_s,
+        "nonfinite_scenes": [i for i, st in enumerate(status) if st],
+    }
+    if tuned:
+        out["autotune"] = tuned
     print(json.dumps(out), flush=True)
 
 

@@ -108,14 +108,18 @@ int32_t mpm_num_particles(const mpm_ctx* ctx);
 
 /* Particle positions, velocities, masses and optional per-particle Young's
  * moduli (NULL: params E); F_E = F_P = I, B = 0, V0 = 0, as
- * initializeParticles leaves them (material_point_method.cpp:18-63). */
+ * initializeParticles leaves them (material_point_method.cpp:18-63).  For a
+ * batch the arrays hold every scene's particles, scene after scene. */
 int mpm_set_particles(mpm_ctx* ctx, const float* x, const float* v, const float* m, const float* E0);
 /* Overwrite any of B, F_E, F_P (9 floats each) and V0; NULL leaves a field. */
 int mpm_set_state(mpm_ctx* ctx, const float* B, const float* FE, const float* FP, const float* V0);
+/* The same colliders for every scene of the context. */
 int mpm_set_colliders(mpm_ctx* ctx, const mpm_collider* c, int32_t k);
 
 /* rasterizeParticlesToGrid + computeParticleVolumesAndDensities
- * (realtime-deformations/main.cpp:53-54). */
+ * (realtime-deformations/main.cpp:53-54), on the device for every scene:
+ * binned mass scatter, node masses, density gathered with the particle's
+ * weights, V0 = m / density (material_point_method.cpp:131-142). */
 int mpm_init_volumes(mpm_ctx* ctx);
 
 /* One legacy stage (MPM_STAGE_*), exactly the reference method's semantics. */
@@ -132,8 +136,41 @@ int mpm_use_graphs(mpm_ctx* ctx, int32_t enable);
 /* Readback into host arrays (NULL skips a field).  getParticles()
  * (material_point_method.hpp:181-183). */
 int mpm_get_particles(mpm_ctx* ctx, float* x, float* v, float* B, float* FE, float* FP, float* V0);
-/* Dense grid: mass, velocity (3), force (3, stage path only), active flag. */
+/* Dense grid of scene 0: mass, velocity (3), force (3, stage path only), active flag. */
 int mpm_get_grid(mpm_ctx* ctx, float* m, float* vel, float* force, uint8_t* active);
+
+/* ---- batched scenes (no reference counterpart: the reference simulates one
+ * scene through globals).  One context steps any number of independent
+ * scenes together: each scene has its own grid (dims, h), material,
+ * gravity, boundary mode and colliders; one binning sort and one launch per
+ * stage cover all of them.  params[0] sets compose and resort_every for the
+ * batch (compose must agree).  Particles are indexed scene after scene;
+ * mpm_num_particles is the total.  The per-stage calls (mpm_stage) need a
+ * single-scene context. ---------------------------------------------- */
+int mpm_create_batch(const mpm_params* params, const int32_t* n_particles, int32_t nscenes, mpm_ctx** out);
+int32_t mpm_num_scenes(const mpm_ctx* ctx);
+int mpm_scene_set_particles(mpm_ctx* ctx, int32_t scene, const float* x, const float* v, const float* m,
+                            const float* E0);
+int mpm_scene_set_state(mpm_ctx* ctx, int32_t scene, const float* B, const float* FE, const float* FP,
+                        const float* V0);
+int mpm_scene_set_colliders(mpm_ctx* ctx, int32_t scene, const mpm_collider* c, int32_t k);
+int mpm_scene_get_particles(mpm_ctx* ctx, int32_t scene, float* x, float* v, float* B, float* FE, float* FP,
+                            float* V0);
+int mpm_scene_get_grid(mpm_ctx* ctx, int32_t scene, float* m, float* vel, float* force, uint8_t* active);
+/* Synchronises and returns, per scene, MPM_OK or the MPM_ERR_* its particles
+ * raised since the last status read (a non-finite scene does not stop the
+ * others); clears the flags. */
+int mpm_scene_status(mpm_ctx* ctx, int32_t* status);
+
+/* ---- launch tuning (no reference counterpart).  resort_every: substeps
+ * between re-binnings; block_grid: workgroups of the persistent block pass.
+ * mpm_autotune times `steps` substeps per candidate (resort_every in
+ * {8, 16, 32} x block_grid in {1024, 2048, 4096}, capped at the block count)
+ * from the current state, restores that state, keeps the fastest and returns
+ * its time per substep. */
+int mpm_set_tuning(mpm_ctx* ctx, int32_t resort_every, int32_t block_grid);
+int mpm_get_tuning(mpm_ctx* ctx, int32_t* resort_every, int32_t* block_grid);
+int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms_per_substep);
 
 /* ---- measurement (no reference counterpart: the reference times stages
  * with millisecond std::chrono stamps, realtime-deformations/utils.h:138-154,

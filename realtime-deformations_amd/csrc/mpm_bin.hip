@@ -2,7 +2,8 @@
 //
 // key(p) = block(p) * 64 + cell-in-block(p), with cell = int(x / h) per axis
 // (correctly rounded FP32 quotient, as material_point_method.cpp:83) and
-// 4x4x4-cell blocks numbered (bi * nb1 + bj) * nb2 + bk.  A stable LSD radix
+// 4x4x4-cell blocks numbered blk0 + (bi * nb1 + bj) * nb2 + bk, blk0 being the
+// first block of the particle's scene: one sort bins every scene.  A stable LSD radix
 // sort (8-bit digits) orders (key, index) pairs exactly like a CPU stable
 // sort: ties keep index order.  Each non-empty block's particles are then
 // moved into one contiguous range, laid out round-robin over the block's 64
@@ -22,15 +23,16 @@ __global__ void __launch_bounds__(256) k_keys(BinArgs b) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p == 0) *b.nblk = 0;
   if (p >= b.n) return;
+  const Scene S = b.scenes[scene_of_slot(b.scenes, b.nscenes, p)];
   int c[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float q = b.P[(int64_t)(FX + a) * b.stride + p] / b.h;
+    const float q = b.P[(int64_t)(FX + a) * b.stride + p] / S.h;
     int v = (int)q;
-    v = v < 1 ? 1 : (v > b.dims[a] - 3 ? b.dims[a] - 3 : v);
+    v = v < 1 ? 1 : (v > S.dims[a] - 3 ? S.dims[a] - 3 : v);
     c[a] = v;
   }
-  const uint32_t blk = (uint32_t)(((c[0] >> 2) * b.nb[1] + (c[1] >> 2)) * b.nb[2] + (c[2] >> 2));
+  const uint32_t blk = (uint32_t)(S.blk0 + ((c[0] >> 2) * S.nb[1] + (c[1] >> 2)) * S.nb[2] + (c[2] >> 2));
   b.keys[p] = blk * 64u + (uint32_t)(((c[0] & 3) << 4) | ((c[1] & 3) << 2) | (c[2] & 3));
   b.vals[p] = (uint32_t)p;
 }
@@ -298,7 +300,7 @@ __global__ void __launch_bounds__(256) k_reorder(BinArgs b) {
         b.pid_out[dst] = b.pid[src];
       }
     }
-    if (t == 0) b.blist[s] = make_int4(blk, start, end, 0);
+    if (t == 0) b.blist[s] = make_int4(blk, start, end, b.blk_scene[blk]);
     __syncthreads();
   }
 }

@@ -11,16 +11,16 @@ struct BinArgs {
   int* pid_out;
   int64_t stride;
   int n;
-  int dims[3];
-  int nb[3];
-  int64_t nnodes;
-  float h;
+  int nscenes;
+  const Scene* scenes;    // device scene table (grids, particle slot ranges)
+  const int* blk_scene;   // grid block -> scene
+  int64_t nnodes;         // nodes over all scenes
   uint32_t *keys, *vals, *keys2, *vals2;  // 4 x n
   uint32_t* hist;                         // 256 x ceil(n / 4096)
   int* blk_start;                         // per grid block
   int* blk_end;
   int* blk_ids;                           // compact list of non-empty blocks
-  int4* blist;                            // (block, start, end, 0) per listed block
+  int4* blist;                            // (block, start, end, scene) per listed block
   int* nblk;                              // number of listed blocks
 };
 

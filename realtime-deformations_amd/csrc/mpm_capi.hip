@@ -97,15 +97,16 @@ template <typename F> int timed(mpm_ctx* ctx, int kernel, F&& launch) {
 }
 
 // Every particle's 5^3 stencil (base-2 .. base+2, base = int(x/h)) must lie
-// inside the grid; the reference indexes out of bounds otherwise.
-int check_positions(mpm_ctx* ctx, const float* x) {
-  for (int p = 0; p < ctx->n; ++p)
+// inside its scene's grid; the reference indexes out of bounds otherwise.
+int check_positions(mpm_ctx* ctx, int scene, const float* x) {
+  const Scene& S = ctx->scenes[scene];
+  for (int p = 0; p < S.np; ++p)
     for (int a = 0; a < 3; ++a) {
-      const float q = x[3 * p + a] / ctx->h;
-      if (!(q >= 2.0f) || !(q < (float)(ctx->dims[a] - 2))) {
-        char buf[160];
-        snprintf(buf, sizeof buf, "particle %d coordinate %d = %g puts its stencil outside the grid", p, a,
-                 (double)x[3 * p + a]);
+      const float q = x[3 * p + a] / S.h;
+      if (!(q >= 2.0f) || !(q < (float)(S.dims[a] - 2))) {
+        char buf[200];
+        snprintf(buf, sizeof buf, "scene %d particle %d coordinate %d = %g puts its stencil outside the grid", scene,
+                 p, a, (double)x[3 * p + a]);
         ctx->err = buf;
         return MPM_ERR_BOUNDS;
       }
@@ -113,19 +114,88 @@ int check_positions(mpm_ctx* ctx, const float* x) {
   return MPM_OK;
 }
 
-int read_status(mpm_ctx* ctx) {
-  int st = 0;
-  MPM_HIP(hipMemcpyAsync(&st, ctx->status, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+int status_code(int st) { return st & ST_NONFINITE ? MPM_ERR_NONFINITE : (st ? MPM_ERR_BOUNDS : MPM_OK); }
+
+// Per-scene device status words: returns (and clears) them; the first failing
+// scene sets the context error.
+int read_status(mpm_ctx* ctx, std::vector<int>* per_scene = nullptr) {
+  std::vector<int> st(ctx->nscenes, 0);
+  MPM_HIP(hipMemcpyAsync(st.data(), ctx->status, sizeof(int) * ctx->nscenes, hipMemcpyDeviceToHost, ctx->stream));
   MPM_HIP(hipStreamSynchronize(ctx->stream));
-  if (st) {
-    MPM_HIP(hipMemsetAsync(ctx->status, 0, sizeof(int), ctx->stream));
-    MPM_HIP(hipStreamSynchronize(ctx->stream));
-    if (st & ST_NONFINITE) {
-      ctx->err = "SVD Error: non-finite deformation gradient (particle F left unchanged)";
-      return MPM_ERR_NONFINITE;
+  if (per_scene) *per_scene = st;
+  int rc = MPM_OK;
+  for (int s = 0; s < ctx->nscenes; ++s)
+    if (st[s] && rc == MPM_OK) {
+      rc = status_code(st[s]);
+      ctx->err = (st[s] & ST_NONFINITE)
+                     ? "SVD Error: non-finite deformation gradient (particle F left unchanged)"
+                     : "device status " + std::to_string(st[s]);
+      if (ctx->nscenes > 1) ctx->err = "scene " + std::to_string(s) + ": " + ctx->err;
     }
-    ctx->err = "device status " + std::to_string(st);
-    return MPM_ERR_BOUNDS;
+  if (rc != MPM_OK) {
+    MPM_HIP(hipMemsetAsync(ctx->status, 0, sizeof(int) * ctx->nscenes, ctx->stream));
+    MPM_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return rc;
+}
+
+// Scene table entry for one mpm_params; grid placement filled by the caller.
+Scene make_scene(const mpm_params& p) {
+  Scene S{};
+  for (int a = 0; a < 3; ++a) {
+    S.dims[a] = p.dims[a];
+    S.nb[a] = (p.dims[a] + 3) / 4;
+    S.g[a] = p.gravity[a];
+  }
+  S.nblocks = (int64_t)S.nb[0] * S.nb[1] * S.nb[2];
+  S.h = p.h;
+  S.inv_h = 1.0f / p.h;
+  S.E = p.E;
+  S.lo = (float)(1.0 - p.theta_c);
+  S.hi = (float)(1.0 + p.theta_s);
+  S.gap64 = p.svd_gap_fp64;
+  S.lame.nu = p.nu;
+  S.lame.xi = p.xi;
+  S.lame.dmu = 2.0f * (1.0f + p.nu);
+  S.lame.rmu = 1.0f / S.lame.dmu;
+  S.lame.dlam = (1.0f + p.nu) * (1.0f - 2.0f * p.nu);
+  S.lame.rlam = 1.0f / S.lame.dlam;
+  S.bc_mode = p.bc_mode;
+  // DpInverse = inverse(mat3(1) * (1/3) * h * h) (material_point_method.hpp:177),
+  // glm compute_inverse on the host
+  float d = 1.0f * (1.0f / 3.0f);
+  d = d * p.h;
+  d = d * p.h;
+  M3 D;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) D.c[i][j] = (i == j) ? d : 0.0f;
+  const float ood = 1.0f / (+D.c[0][0] * (D.c[1][1] * D.c[2][2] - D.c[2][1] * D.c[1][2]) -
+                            D.c[1][0] * (D.c[0][1] * D.c[2][2] - D.c[2][1] * D.c[0][2]) +
+                            D.c[2][0] * (D.c[0][1] * D.c[1][2] - D.c[1][1] * D.c[0][2]));
+  M3 r;
+  r.c[0][0] = +(D.c[1][1] * D.c[2][2] - D.c[2][1] * D.c[1][2]) * ood;
+  r.c[1][0] = -(D.c[1][0] * D.c[2][2] - D.c[2][0] * D.c[1][2]) * ood;
+  r.c[2][0] = +(D.c[1][0] * D.c[2][1] - D.c[2][0] * D.c[1][1]) * ood;
+  r.c[0][1] = -(D.c[0][1] * D.c[2][2] - D.c[2][1] * D.c[0][2]) * ood;
+  r.c[1][1] = +(D.c[0][0] * D.c[2][2] - D.c[2][0] * D.c[0][2]) * ood;
+  r.c[2][1] = -(D.c[0][0] * D.c[2][1] - D.c[2][0] * D.c[0][1]) * ood;
+  r.c[0][2] = +(D.c[0][1] * D.c[1][2] - D.c[1][1] * D.c[0][2]) * ood;
+  r.c[1][2] = -(D.c[0][0] * D.c[1][2] - D.c[1][0] * D.c[0][2]) * ood;
+  r.c[2][2] = +(D.c[0][0] * D.c[1][1] - D.c[1][0] * D.c[0][1]) * ood;
+  S.dinv = r.c[0][0];  // the off-diagonal signed zeros are those of dinv_matrix
+  return S;
+}
+
+int upload_scene_table(mpm_ctx* ctx) {
+  MPM_HIP(hipStreamSynchronize(ctx->stream));
+  MPM_HIP(hipMemcpy(ctx->d_scenes, ctx->scenes.data(), sizeof(Scene) * ctx->nscenes, hipMemcpyHostToDevice));
+  return MPM_OK;
+}
+
+int valid_scene(mpm_ctx* ctx, int scene) {
+  if (scene < 0 || scene >= ctx->nscenes) {
+    ctx->err = "scene index out of range";
+    return MPM_ERR_ARG;
   }
   return MPM_OK;
 }
@@ -141,30 +211,13 @@ KArgs make_args(const mpm_ctx* c, float dt) {
   a.gacc = c->gacc;
   a.gvel = c->gvel;
   a.gforce = c->gforce;
-  for (int i = 0; i < 3; ++i) {
-    a.dims[i] = c->dims[i];
-    a.nb[i] = c->nb[i];
-    a.g[i] = c->prm.gravity[i];
-  }
   a.nnodes = c->nnodes;
-  a.h = c->h;
-  a.inv_h = 1.0f / c->h;
   a.dt = dt;
-  a.lo = c->lo;
-  a.hi = c->hi;
-  a.gap64 = c->prm.svd_gap_fp64;
-  a.Dinv = c->Dinv;
-  const float nu = c->prm.nu;
-  a.lame.nu = nu;
-  a.lame.xi = c->prm.xi;
-  a.lame.dmu = 2.0f * (1.0f + nu);
-  a.lame.rmu = 1.0f / a.lame.dmu;
-  a.lame.dlam = (1.0f + nu) * (1.0f - 2.0f * nu);
-  a.lame.rlam = 1.0f / a.lame.dlam;
   a.compose = c->prm.compose;
-  a.bc_mode = c->prm.bc_mode;
-  a.ncoll = c->ncoll;
-  a.coll = c->coll;
+  a.nscenes = c->nscenes;
+  a.scenes = c->d_scenes;
+  a.blk_scene = c->blk_scene;
+  a.sc = c->scenes[0];
   a.status = c->status;
   return a;
 }
@@ -215,58 +268,56 @@ void mpm_default_params(mpm_params* p, int32_t ni, int32_t nj, int32_t nk, float
   p->svd_gap_fp64 = 0.0f;
 }
 
-int mpm_create(const mpm_params* p, int32_t n, mpm_ctx** out) {
-  if (!p || !out || n < 0) return MPM_ERR_ARG;
-  for (int a = 0; a < 3; ++a)
-    if (p->dims[a] < 6) return MPM_ERR_ARG;
-  if (!(p->h > 0.0f)) return MPM_ERR_ARG;
-  if (p->compose != MPM_COMPOSE_LITERAL && p->compose != MPM_COMPOSE_TEXTBOOK) return MPM_ERR_ARG;
+int mpm_create_batch(const mpm_params* params, const int32_t* n_particles, int32_t nscenes, mpm_ctx** out) {
+  if (!params || !n_particles || !out || nscenes < 1) return MPM_ERR_ARG;
+  int64_t ntot = 0;
+  for (int s = 0; s < nscenes; ++s) {
+    const mpm_params& p = params[s];
+    if (n_particles[s] < 0) return MPM_ERR_ARG;
+    for (int a = 0; a < 3; ++a)
+      if (p.dims[a] < 6) return MPM_ERR_ARG;
+    if (!(p.h > 0.0f)) return MPM_ERR_ARG;
+    if (p.compose != MPM_COMPOSE_LITERAL && p.compose != MPM_COMPOSE_TEXTBOOK) return MPM_ERR_ARG;
+    if (p.compose != params[0].compose) return MPM_ERR_ARG;  // one fused kernel per launch
+    ntot += n_particles[s];
+  }
+  if (ntot >= (int64_t)1 << 30) return MPM_ERR_ARG;  // slot byte offsets are 32-bit
   auto* ctx = new (std::nothrow) mpm_ctx();
   if (!ctx) return MPM_ERR_NOMEM;
   *out = ctx;
-  ctx->prm = *p;
-  ctx->n = n;
-  ctx->stride = ((int64_t)n + 63) / 64 * 64;
+  ctx->prm = params[0];
+  ctx->nscenes = nscenes;
+  ctx->scenes.resize(nscenes);
+  ctx->scoll.assign(nscenes, nullptr);
+  ctx->scene_set.assign(nscenes, 0);
+  int64_t blk0 = 0;
+  int p0 = 0;
+  for (int s = 0; s < nscenes; ++s) {
+    Scene& S = ctx->scenes[s];
+    S = make_scene(params[s]);
+    S.blk0 = blk0;
+    S.p0 = p0;
+    S.np = n_particles[s];
+    blk0 += S.nblocks;
+    p0 += S.np;
+  }
+  ctx->nblocks = blk0;
+  ctx->nnodes = blk0 * 64;
+  if (ctx->nnodes >= ((int64_t)1 << 32)) {  // sort keys are block * 64 + cell in 32 bits
+    ctx->err = "grids too large for 32-bit sort keys";
+    return MPM_ERR_ARG;
+  }
+  ctx->n = (int)ntot;
+  ctx->stride = ((int64_t)ctx->n + 63) / 64 * 64;
   if (ctx->stride == 0) ctx->stride = 64;
-  for (int a = 0; a < 3; ++a) {
-    ctx->dims[a] = p->dims[a];
-    ctx->nb[a] = (p->dims[a] + 3) / 4;
-  }
-  ctx->nnodes = (int64_t)ctx->nb[0] * ctx->nb[1] * ctx->nb[2] * 64;
-  ctx->h = p->h;
-  ctx->lo = (float)(1.0 - p->theta_c);
-  ctx->hi = (float)(1.0 + p->theta_s);
-  // DpInverse = inverse(mat3(1) * (1/3) * h * h) (material_point_method.hpp:177)
-  {
-    float d = 1.0f * (1.0f / 3.0f);
-    d = d * p->h;
-    d = d * p->h;
-    M3 D;
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) D.c[i][j] = (i == j) ? d : 0.0f;
-    // glm compute_inverse on the host
-    const float ood = 1.0f / (+D.c[0][0] * (D.c[1][1] * D.c[2][2] - D.c[2][1] * D.c[1][2]) -
-                              D.c[1][0] * (D.c[0][1] * D.c[2][2] - D.c[2][1] * D.c[0][2]) +
-                              D.c[2][0] * (D.c[0][1] * D.c[1][2] - D.c[1][1] * D.c[0][2]));
-    M3 r;
-    r.c[0][0] = +(D.c[1][1] * D.c[2][2] - D.c[2][1] * D.c[1][2]) * ood;
-    r.c[1][0] = -(D.c[1][0] * D.c[2][2] - D.c[2][0] * D.c[1][2]) * ood;
-    r.c[2][0] = +(D.c[1][0] * D.c[2][1] - D.c[2][0] * D.c[1][1]) * ood;
-    r.c[0][1] = -(D.c[0][1] * D.c[2][2] - D.c[2][1] * D.c[0][2]) * ood;
-    r.c[1][1] = +(D.c[0][0] * D.c[2][2] - D.c[2][0] * D.c[0][2]) * ood;
-    r.c[2][1] = -(D.c[0][0] * D.c[2][1] - D.c[2][0] * D.c[0][1]) * ood;
-    r.c[0][2] = +(D.c[0][1] * D.c[1][2] - D.c[1][1] * D.c[0][2]) * ood;
-    r.c[1][2] = -(D.c[0][0] * D.c[1][2] - D.c[1][0] * D.c[0][2]) * ood;
-    r.c[2][2] = +(D.c[0][0] * D.c[1][1] - D.c[1][0] * D.c[0][1]) * ood;
-    ctx->Dinv = r;
-  }
-  ctx->resort_every = p->resort_every > 0 ? p->resort_every : 16;
+  ctx->resort_every = params[0].resort_every > 0 ? params[0].resort_every : 16;
+  ctx->block_grid = (int)std::max<int64_t>(1, std::min<int64_t>(ctx->nblocks, 2048));
   MPM_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   const size_t pbytes = sizeof(float) * (size_t)NFIELDS * (size_t)ctx->stride;
   const size_t gbytes = sizeof(float4) * (size_t)ctx->nnodes;
-  const int64_t nblocks = (int64_t)ctx->nb[0] * ctx->nb[1] * ctx->nb[2];
-  const int64_t ntiles = ((int64_t)n + 4095) / 4096;
-  const size_t nn = (size_t)std::max(1, n);
+  const int64_t ntiles = ((int64_t)ctx->n + 4095) / 4096;
+  const size_t nn = (size_t)std::max(1, ctx->n);
+  const size_t nb = (size_t)ctx->nblocks;
   bool ok = true;
   auto alloc = [&](auto** ptr, size_t bytes) {
     if (ok && hipMalloc(reinterpret_cast<void**>(ptr), bytes) != hipSuccess) ok = false;
@@ -280,39 +331,49 @@ int mpm_create(const mpm_params* p, int32_t n, mpm_ctx** out) {
   alloc(&ctx->keys2, sizeof(uint32_t) * nn);
   alloc(&ctx->vals2, sizeof(uint32_t) * nn);
   alloc(&ctx->hist, sizeof(uint32_t) * 256 * (size_t)(std::max<int64_t>(1, ntiles) + 1));
-  alloc(&ctx->blk_start, sizeof(int) * nblocks);
-  alloc(&ctx->blk_end, sizeof(int) * nblocks);
-  alloc(&ctx->blk_ids, sizeof(int) * nblocks);
-  alloc(&ctx->blist, sizeof(int4) * nblocks);
+  alloc(&ctx->blk_start, sizeof(int) * nb);
+  alloc(&ctx->blk_end, sizeof(int) * nb);
+  alloc(&ctx->blk_ids, sizeof(int) * nb);
+  alloc(&ctx->blist, sizeof(int4) * nb);
   alloc(&ctx->nblk, sizeof(int));
-  alloc(&ctx->gflag, sizeof(int) * nblocks);
-  alloc(&ctx->glist[0], sizeof(int) * nblocks);
-  alloc(&ctx->glist[1], sizeof(int) * nblocks);
+  alloc(&ctx->gflag, sizeof(int) * nb);
+  alloc(&ctx->glist[0], sizeof(int) * nb);
+  alloc(&ctx->glist[1], sizeof(int) * nb);
   alloc(&ctx->gcnt, sizeof(int) * 4);
   alloc(&ctx->dstep, sizeof(int));
   alloc(&ctx->gacc, gbytes);
   alloc(&ctx->gvel, gbytes);
   alloc(&ctx->gforce, gbytes);
-  alloc(&ctx->status, sizeof(int));
+  alloc(&ctx->status, sizeof(int) * nscenes);
+  alloc(&ctx->d_scenes, sizeof(Scene) * nscenes);
+  alloc(&ctx->blk_scene, sizeof(int) * nb);
   if (!ok) {
     ctx->err = "device allocation failed";
     return MPM_ERR_NOMEM;
   }
   MPM_HIP(hipMemsetAsync(ctx->P, 0, pbytes, ctx->stream));
   MPM_HIP(hipMemsetAsync(ctx->P2, 0, pbytes, ctx->stream));
-  launch_iota(ctx->pid, n, ctx->stream);
-  launch_fill_int(ctx->gflag, nblocks, 0, ctx->stream);
+  launch_iota(ctx->pid, ctx->n, ctx->stream);
+  for (int s = 0; s < nscenes; ++s)
+    launch_fill_int(ctx->blk_scene + ctx->scenes[s].blk0, ctx->scenes[s].nblocks, s, ctx->stream);
+  launch_fill_int(ctx->gflag, ctx->nblocks, 0, ctx->stream);
   launch_fill_int(ctx->gcnt, 4, 0, ctx->stream);
   launch_fill_int(ctx->dstep, 1, 1, ctx->stream);
   launch_fill_int(ctx->nblk, 1, 0, ctx->stream);
   launch_clear(ctx->gacc, ctx->nnodes, ctx->stream);
   launch_clear(ctx->gvel, ctx->nnodes, ctx->stream);
   launch_clear(ctx->gforce, ctx->nnodes, ctx->stream);
-  MPM_HIP(hipMemsetAsync(ctx->status, 0, sizeof(int), ctx->stream));
+  MPM_HIP(hipMemsetAsync(ctx->status, 0, sizeof(int) * nscenes, ctx->stream));
+  if (int rc = upload_scene_table(ctx)) return rc;
   MPM_HIP(hipStreamSynchronize(ctx->stream));
   ctx->gacc_clean = true;
   ctx->P_first = ctx->P;
   return MPM_OK;
+}
+
+int mpm_create(const mpm_params* p, int32_t n, mpm_ctx** out) {
+  if (!p || !out || n < 0) return MPM_ERR_ARG;
+  return mpm_create_batch(p, &n, 1, out);
 }
 
 int mpm_destroy(mpm_ctx* ctx) {
@@ -323,8 +384,9 @@ int mpm_destroy(mpm_ctx* ctx) {
   void* bufs[] = {ctx->P,       ctx->P2,      ctx->pid,      ctx->pid2,  ctx->keys,     ctx->vals,     ctx->keys2,
                   ctx->vals2,   ctx->hist,    ctx->blk_start, ctx->blk_end, ctx->blk_ids, ctx->blist,   ctx->nblk,
                   ctx->gflag,   ctx->glist[0], ctx->glist[1], ctx->gcnt,  ctx->dstep,    ctx->gacc,     ctx->gvel,
-                  ctx->gforce,  ctx->coll,    ctx->status,  ctx->dcount};
+                  ctx->gforce,  ctx->status,  ctx->dcount,  ctx->d_scenes, ctx->blk_scene};
   for (void* b : bufs) (void)hipFree(b);
+  for (DevCollider* c : ctx->scoll) (void)hipFree(c);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->ev_region)
     if (e) (void)hipEventDestroy(e);
@@ -335,66 +397,106 @@ int mpm_destroy(mpm_ctx* ctx) {
 
 const char* mpm_last_error(const mpm_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 int32_t mpm_num_particles(const mpm_ctx* ctx) { return ctx ? ctx->n : 0; }
+int32_t mpm_num_scenes(const mpm_ctx* ctx) { return ctx ? ctx->nscenes : 0; }
 
-static int upload_fields(mpm_ctx* ctx, int field, const float* host, int comps) {
-  if (!host || ctx->n == 0) return MPM_OK;
+static int upload_fields(mpm_ctx* ctx, int scene, int field, const float* host, int comps) {
+  const Scene& S = ctx->scenes[scene];
+  if (!host || S.np == 0) return MPM_OK;
   float* tmp = nullptr;
-  const size_t bytes = sizeof(float) * (size_t)ctx->n * comps;
+  const size_t bytes = sizeof(float) * (size_t)S.np * comps;
   MPM_HIP(hipMalloc(&tmp, bytes));
   MPM_HIP(hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, ctx->stream));
-  launch_scatter_field(ctx->P + (int64_t)field * ctx->stride, ctx->stride, tmp, ctx->pid, ctx->n, comps, ctx->stream);
+  launch_scatter_field(ctx->P + (int64_t)field * ctx->stride, ctx->stride, tmp, ctx->pid, S.p0, S.np, comps,
+                       ctx->stream);
   MPM_HIP(hipStreamSynchronize(ctx->stream));
   MPM_HIP(hipFree(tmp));
   return MPM_OK;
 }
-static int download_fields(mpm_ctx* ctx, int field, float* host, int comps) {
-  if (!host || ctx->n == 0) return MPM_OK;
+static int download_fields(mpm_ctx* ctx, int scene, int field, float* host, int comps) {
+  const Scene& S = ctx->scenes[scene];
+  if (!host || S.np == 0) return MPM_OK;
   float* tmp = nullptr;
-  const size_t bytes = sizeof(float) * (size_t)ctx->n * comps;
+  const size_t bytes = sizeof(float) * (size_t)S.np * comps;
   MPM_HIP(hipMalloc(&tmp, bytes));
-  launch_gather_field(tmp, ctx->P + (int64_t)field * ctx->stride, ctx->stride, ctx->pid, ctx->n, comps, ctx->stream);
+  launch_gather_field(tmp, ctx->P + (int64_t)field * ctx->stride, ctx->stride, ctx->pid, S.p0, S.np, comps,
+                      ctx->stream);
   MPM_HIP(hipMemcpyAsync(host, tmp, bytes, hipMemcpyDeviceToHost, ctx->stream));
   MPM_HIP(hipStreamSynchronize(ctx->stream));
   MPM_HIP(hipFree(tmp));
   return MPM_OK;
 }
 
-int mpm_set_particles(mpm_ctx* ctx, const float* x, const float* v, const float* m, const float* E0) {
+int mpm_scene_set_particles(mpm_ctx* ctx, int32_t scene, const float* x, const float* v, const float* m,
+                            const float* E0) {
   if (check_ctx(ctx)) return MPM_ERR_ARG;
-  if (ctx->n && (!x || !v || !m)) {
+  if (int rc = valid_scene(ctx, scene)) return rc;
+  const Scene& S = ctx->scenes[scene];
+  if (S.np && (!x || !v || !m)) {
     ctx->err = "x, v and m are required";
     return MPM_ERR_ARG;
   }
-  if (int rc = check_positions(ctx, x)) return rc;
-  launch_iota(ctx->pid, ctx->n, ctx->stream);  // caller order again
+  if (int rc = check_positions(ctx, scene, x)) return rc;
+  // caller order again inside this scene's slots; every scene re-bins next
+  MPM_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<int> ids((size_t)S.np);
+  for (int i = 0; i < S.np; ++i) ids[i] = S.p0 + i;
+  if (S.np) MPM_HIP(hipMemcpy(ctx->pid + S.p0, ids.data(), sizeof(int) * S.np, hipMemcpyHostToDevice));
   ctx->since_sort = -1;
   int rc;
-  if ((rc = upload_fields(ctx, FX, x, 3)) || (rc = upload_fields(ctx, FV, v, 3)) || (rc = upload_fields(ctx, FM, m, 1)))
+  if ((rc = upload_fields(ctx, scene, FX, x, 3)) || (rc = upload_fields(ctx, scene, FV, v, 3)) ||
+      (rc = upload_fields(ctx, scene, FM, m, 1)))
     return rc;
-  std::vector<float> ident((size_t)ctx->n * 9, 0.0f), zeros((size_t)ctx->n * 9, 0.0f), e((size_t)ctx->n);
-  for (int p = 0; p < ctx->n; ++p) {
+  std::vector<float> ident((size_t)S.np * 9, 0.0f), zeros((size_t)S.np * 9, 0.0f), e((size_t)S.np);
+  for (int p = 0; p < S.np; ++p) {
     ident[9 * p + 0] = ident[9 * p + 4] = ident[9 * p + 8] = 1.0f;
-    e[p] = E0 ? E0[p] : ctx->prm.E;
+    e[p] = E0 ? E0[p] : S.E;
   }
-  if ((rc = upload_fields(ctx, FB, zeros.data(), 9)) || (rc = upload_fields(ctx, FFE, ident.data(), 9)) ||
-      (rc = upload_fields(ctx, FFP, ident.data(), 9)) || (rc = upload_fields(ctx, FVOL, zeros.data(), 1)) ||
-      (rc = upload_fields(ctx, FE0, e.data(), 1)))
+  if ((rc = upload_fields(ctx, scene, FB, zeros.data(), 9)) || (rc = upload_fields(ctx, scene, FFE, ident.data(), 9)) ||
+      (rc = upload_fields(ctx, scene, FFP, ident.data(), 9)) || (rc = upload_fields(ctx, scene, FVOL, zeros.data(), 1)) ||
+      (rc = upload_fields(ctx, scene, FE0, e.data(), 1)))
     return rc;
+  ctx->scene_set[scene] = 1;
   ctx->particles_set = true;
+  for (char c : ctx->scene_set) ctx->particles_set = ctx->particles_set && c;
+  return MPM_OK;
+}
+
+int mpm_set_particles(mpm_ctx* ctx, const float* x, const float* v, const float* m, const float* E0) {
+  if (check_ctx(ctx)) return MPM_ERR_ARG;
+  for (int s = 0; s < ctx->nscenes; ++s) {
+    const int p0 = ctx->scenes[s].p0;
+    if (int rc = mpm_scene_set_particles(ctx, s, x ? x + 3 * (int64_t)p0 : nullptr, v ? v + 3 * (int64_t)p0 : nullptr,
+                                         m ? m + p0 : nullptr, E0 ? E0 + p0 : nullptr))
+      return rc;
+  }
+  return MPM_OK;
+}
+
+int mpm_scene_set_state(mpm_ctx* ctx, int32_t scene, const float* B, const float* FE, const float* FP,
+                        const float* V0) {
+  if (check_ctx(ctx)) return MPM_ERR_ARG;
+  if (int rc = valid_scene(ctx, scene)) return rc;
+  int rc;
+  if ((rc = upload_fields(ctx, scene, FB, B, 9)) || (rc = upload_fields(ctx, scene, FFE, FE, 9)) ||
+      (rc = upload_fields(ctx, scene, FFP, FP, 9)) || (rc = upload_fields(ctx, scene, FVOL, V0, 1)))
+    return rc;
   return MPM_OK;
 }
 
 int mpm_set_state(mpm_ctx* ctx, const float* B, const float* FE, const float* FP, const float* V0) {
   if (check_ctx(ctx)) return MPM_ERR_ARG;
-  int rc;
-  if ((rc = upload_fields(ctx, FB, B, 9)) || (rc = upload_fields(ctx, FFE, FE, 9)) ||
-      (rc = upload_fields(ctx, FFP, FP, 9)) || (rc = upload_fields(ctx, FVOL, V0, 1)))
-    return rc;
+  for (int s = 0; s < ctx->nscenes; ++s) {
+    const int64_t p0 = ctx->scenes[s].p0;
+    if (int rc = mpm_scene_set_state(ctx, s, B ? B + 9 * p0 : nullptr, FE ? FE + 9 * p0 : nullptr,
+                                     FP ? FP + 9 * p0 : nullptr, V0 ? V0 + p0 : nullptr))
+      return rc;
+  }
   return MPM_OK;
 }
 
-int mpm_set_colliders(mpm_ctx* ctx, const mpm_collider* c, int32_t k) {
+int mpm_scene_set_colliders(mpm_ctx* ctx, int32_t scene, const mpm_collider* c, int32_t k) {
   if (check_ctx(ctx) || k < 0 || (k > 0 && !c)) return MPM_ERR_ARG;
+  if (int rc = valid_scene(ctx, scene)) return rc;
   std::vector<DevCollider> host((size_t)k);
   for (int i = 0; i < k; ++i) {
     host_collider_inverse(c[i].translation, c[i].rotation, host[i].inv);
@@ -404,13 +506,22 @@ int mpm_set_colliders(mpm_ctx* ctx, const mpm_collider* c, int32_t k) {
     }
   }
   MPM_HIP(hipStreamSynchronize(ctx->stream));
-  (void)hipFree(ctx->coll);
-  ctx->coll = nullptr;
-  ctx->ncoll = k;
+  DevCollider*& dc = ctx->scoll[scene];
+  (void)hipFree(dc);
+  dc = nullptr;
   if (k) {
-    MPM_HIP(hipMalloc(&ctx->coll, sizeof(DevCollider) * k));
-    MPM_HIP(hipMemcpy(ctx->coll, host.data(), sizeof(DevCollider) * k, hipMemcpyHostToDevice));
+    MPM_HIP(hipMalloc(&dc, sizeof(DevCollider) * k));
+    MPM_HIP(hipMemcpy(dc, host.data(), sizeof(DevCollider) * k, hipMemcpyHostToDevice));
   }
+  ctx->scenes[scene].coll = dc;
+  ctx->scenes[scene].ncoll = k;
+  return upload_scene_table(ctx);
+}
+
+int mpm_set_colliders(mpm_ctx* ctx, const mpm_collider* c, int32_t k) {
+  if (check_ctx(ctx)) return MPM_ERR_ARG;
+  for (int s = 0; s < ctx->nscenes; ++s)
+    if (int rc = mpm_scene_set_colliders(ctx, s, c, k)) return rc;
   return MPM_OK;
 }
 
@@ -418,6 +529,10 @@ int mpm_stage(mpm_ctx* ctx, int32_t stage, float dt) {
   if (check_ctx(ctx)) return MPM_ERR_ARG;
   if (!ctx->particles_set) {
     ctx->err = "no particles set";
+    return MPM_ERR_STATE;
+  }
+  if (ctx->nscenes != 1) {
+    ctx->err = "the per-stage calls run single-scene contexts only";
     return MPM_ERR_STATE;
   }
   const KArgs a = make_args(ctx, dt);
@@ -443,11 +558,7 @@ int mpm_stage(mpm_ctx* ctx, int32_t stage, float dt) {
   return read_status(ctx);
 }
 
-int mpm_init_volumes(mpm_ctx* ctx) {
-  int rc = mpm_stage(ctx, MPM_STAGE_P2G, 0.0f);
-  if (rc) return rc;
-  return mpm_stage(ctx, MPM_STAGE_VOLUMES, 0.0f);
-}
+}  // extern "C"
 
 namespace {
 
@@ -461,6 +572,8 @@ BlockArgs block_args(mpm_ctx* ctx, float dt) {
   ba.glist[1] = ctx->glist[1];
   ba.gcnt = ctx->gcnt;
   ba.dstep = ctx->dstep;
+  ba.nblocks = ctx->nblocks;
+  ba.grid = ctx->block_grid;
   return ba;
 }
 
@@ -472,12 +585,10 @@ BinArgs bin_args(mpm_ctx* ctx) {
   b.pid_out = ctx->pid2;
   b.stride = ctx->stride;
   b.n = ctx->n;
-  for (int i = 0; i < 3; ++i) {
-    b.dims[i] = ctx->dims[i];
-    b.nb[i] = ctx->nb[i];
-  }
+  b.nscenes = ctx->nscenes;
+  b.scenes = ctx->d_scenes;
+  b.blk_scene = ctx->blk_scene;
   b.nnodes = ctx->nnodes;
-  b.h = ctx->h;
   b.keys = ctx->keys;
   b.vals = ctx->vals;
   b.keys2 = ctx->keys2;
@@ -502,6 +613,18 @@ int rebin(mpm_ctx* ctx) {
   return MPM_OK;
 }
 
+// Grid accumulators, velocities and block lists back to their clean state
+// (after the legacy stages or the initial-volume pass used them).
+void reset_grid(mpm_ctx* ctx) {
+  hipStream_t s = ctx->stream;
+  launch_clear(ctx->gacc, ctx->nnodes, s);
+  launch_clear(ctx->gvel, ctx->nnodes, s);
+  launch_fill_int(ctx->gflag, ctx->nblocks, 0, s);
+  launch_fill_int(ctx->gcnt, 4, 0, s);
+  launch_fill_int(ctx->dstep, 1, 1, s);
+  ctx->gacc_clean = true;
+}
+
 // One substep: re-bin when due, then the block pass (PASS_P2G on the first
 // substep of a call, PASS_FUSED after it), the grid pass and the epoch advance.
 int substep(mpm_ctx* ctx, float dt, bool first) {
@@ -521,14 +644,15 @@ int substep(mpm_ctx* ctx, float dt, bool first) {
 
 // resort_every fused substeps that start with a re-bin, replayed from a HIP
 // graph.  The launches bake in the particle buffers (which alternate with
-// every re-bin, hence one graph per buffer parity), dt and the collider
-// table; the substep epoch and block lists are device state, so one graph
-// serves every segment.
+// every re-bin, hence one graph per buffer parity), dt, the segment length
+// and the block-pass width; scene data (colliders included), the substep
+// epoch and the block lists are device state, so one graph serves every
+// segment.
 int graph_segment(mpm_ctx* ctx, float dt) {
   const int par = ctx->P == ctx->P_first ? 0 : 1;
+  const int key = ctx->resort_every * 65536 + ctx->block_grid;
   hipGraphExec_t& g = ctx->gexec[par];
-  if (g && (ctx->graph_dt[par] != dt || ctx->graph_coll[par] != (const void*)ctx->coll ||
-            ctx->graph_ncoll[par] != ctx->ncoll)) {
+  if (g && (ctx->graph_dt[par] != dt || ctx->graph_key[par] != key)) {
     MPM_HIP(hipGraphExecDestroy(g));
     g = nullptr;
   }
@@ -556,8 +680,7 @@ int graph_segment(mpm_ctx* ctx, float dt) {
     (void)hipGraphDestroy(graph);
     MPM_HIP(ie);
     ctx->graph_dt[par] = dt;
-    ctx->graph_coll[par] = ctx->coll;
-    ctx->graph_ncoll[par] = ctx->ncoll;
+    ctx->graph_key[par] = key;
   }
   MPM_HIP(hipGraphLaunch(g, ctx->stream));
   std::swap(ctx->P, ctx->P2);
@@ -568,6 +691,30 @@ int graph_segment(mpm_ctx* ctx, float dt) {
 
 }  // namespace
 
+extern "C" {
+
+// rasterizeParticlesToGrid + computeParticleVolumesAndDensities for every
+// scene on the device: bin, scatter the masses through the block pass,
+// node mass, then each particle's density gathered with its weights.
+int mpm_init_volumes(mpm_ctx* ctx) {
+  if (check_ctx(ctx)) return MPM_ERR_ARG;
+  if (!ctx->particles_set) {
+    ctx->err = "no particles set";
+    return MPM_ERR_STATE;
+  }
+  if (ctx->n == 0) return MPM_OK;
+  int rc;
+  if (!ctx->gacc_clean) reset_grid(ctx);
+  if ((rc = rebin(ctx))) return rc;
+  const BlockArgs ba = block_args(ctx, 0.0f);
+  launch_block_pass(ba, PASS_MASS, ctx->stream);
+  launch_stage_normalise(ba.a, ctx->stream);
+  launch_volumes(ba.a, ctx->stream);
+  ctx->gacc_clean = false;
+  MPM_HIP(hipGetLastError());
+  return read_status(ctx);
+}
+
 int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps) {
   if (check_ctx(ctx) || nsteps < 0) return MPM_ERR_ARG;
   if (!ctx->particles_set) {
@@ -576,16 +723,7 @@ int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps) {
   }
   if (nsteps == 0 || ctx->n == 0) return MPM_OK;
   hipStream_t s = ctx->stream;
-  if (!ctx->gacc_clean) {
-    // entering the fused path after legacy stages: clean grid and block lists
-    const int64_t nblocks = (int64_t)ctx->nb[0] * ctx->nb[1] * ctx->nb[2];
-    launch_clear(ctx->gacc, ctx->nnodes, s);
-    launch_clear(ctx->gvel, ctx->nnodes, s);
-    launch_fill_int(ctx->gflag, nblocks, 0, s);
-    launch_fill_int(ctx->gcnt, 4, 0, s);
-    launch_fill_int(ctx->dstep, 1, 1, s);
-    ctx->gacc_clean = true;
-  }
+  if (!ctx->gacc_clean) reset_grid(ctx);
   const bool graphs = ctx->use_graphs && !(ctx->profiling && ctx->per_kernel);
   int rc;
   for (int i = 0; i < nsteps;) {
@@ -606,6 +744,97 @@ int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps) {
 int mpm_use_graphs(mpm_ctx* ctx, int32_t enable) {
   if (check_ctx(ctx)) return MPM_ERR_ARG;
   ctx->use_graphs = enable != 0;
+  return MPM_OK;
+}
+
+int mpm_set_tuning(mpm_ctx* ctx, int32_t resort_every, int32_t block_grid) {
+  if (check_ctx(ctx) || resort_every < 1 || block_grid < 1) return MPM_ERR_ARG;
+  ctx->resort_every = resort_every;
+  ctx->block_grid = block_grid;
+  return MPM_OK;
+}
+
+int mpm_get_tuning(mpm_ctx* ctx, int32_t* resort_every, int32_t* block_grid) {
+  if (check_ctx(ctx)) return MPM_ERR_ARG;
+  if (resort_every) *resort_every = ctx->resort_every;
+  if (block_grid) *block_grid = ctx->block_grid;
+  return MPM_OK;
+}
+
+// Times `steps` substeps for every candidate (re-bin interval x block-pass
+// width) from the same starting state, keeps the fastest, and puts the
+// particle state back as it was.
+int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms) {
+  if (check_ctx(ctx) || steps < 1) return MPM_ERR_ARG;
+  if (!ctx->particles_set) {
+    ctx->err = "no particles set";
+    return MPM_ERR_STATE;
+  }
+  const size_t pbytes = sizeof(float) * (size_t)NFIELDS * (size_t)ctx->stride;
+  float* snapP = nullptr;
+  int* snapId = nullptr;
+  MPM_HIP(hipMalloc(&snapP, pbytes));
+  MPM_HIP(hipMalloc(&snapId, sizeof(int) * (size_t)std::max(1, ctx->n)));
+  MPM_HIP(hipMemcpyAsync(snapP, ctx->P, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
+  MPM_HIP(hipMemcpyAsync(snapId, ctx->pid, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
+  float* const snapBuf = ctx->P;  // the buffer the snapshot belongs in
+  hipEvent_t e0, e1;
+  MPM_HIP(hipEventCreate(&e0));
+  MPM_HIP(hipEventCreate(&e1));
+  const int resorts[] = {8, 16, 32};
+  const int64_t wmax = std::max<int64_t>(1, ctx->nblocks);
+  const int widths[] = {(int)std::min<int64_t>(wmax, 1024), (int)std::min<int64_t>(wmax, 2048),
+                        (int)std::min<int64_t>(wmax, 4096)};
+  const int keep_r = ctx->resort_every, keep_w = ctx->block_grid;
+  double best = 1e300;
+  int best_r = keep_r, best_w = keep_w, rc = MPM_OK;
+  for (int r : resorts) {
+    for (int w : widths) {
+      // restore the snapshot into the buffer it came from
+      if (ctx->P != snapBuf) {
+        std::swap(ctx->P, ctx->P2);
+        std::swap(ctx->pid, ctx->pid2);
+      }
+      MPM_HIP(hipMemcpyAsync(ctx->P, snapP, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
+      MPM_HIP(hipMemcpyAsync(ctx->pid, snapId, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
+      ctx->since_sort = -1;
+      ctx->resort_every = r;
+      ctx->block_grid = w;
+      MPM_HIP(hipEventRecord(e0, ctx->stream));
+      if ((rc = mpm_substeps(ctx, dt, steps))) break;
+      MPM_HIP(hipEventRecord(e1, ctx->stream));
+      MPM_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      MPM_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_r = r;
+        best_w = w;
+      }
+    }
+    if (rc) break;
+  }
+  if (ctx->P != snapBuf) {
+    std::swap(ctx->P, ctx->P2);
+    std::swap(ctx->pid, ctx->pid2);
+  }
+  MPM_HIP(hipMemcpyAsync(ctx->P, snapP, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
+  MPM_HIP(hipMemcpyAsync(ctx->pid, snapId, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
+  MPM_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->since_sort = -1;
+  ctx->gacc_clean = false;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(snapP);
+  (void)hipFree(snapId);
+  if (rc) {
+    ctx->resort_every = keep_r;
+    ctx->block_grid = keep_w;
+    return rc;
+  }
+  ctx->resort_every = best_r;
+  ctx->block_grid = best_w;
+  if (best_ms) *best_ms = best / steps;
   return MPM_OK;
 }
 
@@ -666,26 +895,51 @@ int mpm_sync(mpm_ctx* ctx) {
   return read_status(ctx);
 }
 
+int mpm_scene_status(mpm_ctx* ctx, int32_t* status) {
+  if (check_ctx(ctx) || !status) return MPM_ERR_ARG;
+  MPM_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<int> st;
+  const int rc = read_status(ctx, &st);
+  for (int s = 0; s < ctx->nscenes; ++s) status[s] = status_code(st[s]);
+  return rc == MPM_ERR_HIP ? rc : MPM_OK;
+}
+
+int mpm_scene_get_particles(mpm_ctx* ctx, int32_t scene, float* x, float* v, float* B, float* FE, float* FP,
+                            float* V0) {
+  if (check_ctx(ctx)) return MPM_ERR_ARG;
+  if (int rc = valid_scene(ctx, scene)) return rc;
+  int rc;
+  if ((rc = download_fields(ctx, scene, FX, x, 3)) || (rc = download_fields(ctx, scene, FV, v, 3)) ||
+      (rc = download_fields(ctx, scene, FB, B, 9)) || (rc = download_fields(ctx, scene, FFE, FE, 9)) ||
+      (rc = download_fields(ctx, scene, FFP, FP, 9)) || (rc = download_fields(ctx, scene, FVOL, V0, 1)))
+    return rc;
+  return MPM_OK;
+}
+
 int mpm_get_particles(mpm_ctx* ctx, float* x, float* v, float* B, float* FE, float* FP, float* V0) {
   if (check_ctx(ctx)) return MPM_ERR_ARG;
-  int rc;
-  if ((rc = download_fields(ctx, FX, x, 3)) || (rc = download_fields(ctx, FV, v, 3)) ||
-      (rc = download_fields(ctx, FB, B, 9)) || (rc = download_fields(ctx, FFE, FE, 9)) ||
-      (rc = download_fields(ctx, FFP, FP, 9)) || (rc = download_fields(ctx, FVOL, V0, 1)))
-    return rc;
+  for (int s = 0; s < ctx->nscenes; ++s) {
+    const int64_t p0 = ctx->scenes[s].p0;
+    if (int rc = mpm_scene_get_particles(ctx, s, x ? x + 3 * p0 : nullptr, v ? v + 3 * p0 : nullptr,
+                                         B ? B + 9 * p0 : nullptr, FE ? FE + 9 * p0 : nullptr,
+                                         FP ? FP + 9 * p0 : nullptr, V0 ? V0 + p0 : nullptr))
+      return rc;
+  }
   return read_status(ctx);
 }
 
-int mpm_get_grid(mpm_ctx* ctx, float* m, float* vel, float* force, uint8_t* active) {
+int mpm_scene_get_grid(mpm_ctx* ctx, int32_t scene, float* m, float* vel, float* force, uint8_t* active) {
   if (check_ctx(ctx)) return MPM_ERR_ARG;
-  const int64_t cells = (int64_t)ctx->dims[0] * ctx->dims[1] * ctx->dims[2];
+  if (int rc = valid_scene(ctx, scene)) return rc;
+  const Scene& S = ctx->scenes[scene];
+  const int64_t cells = (int64_t)S.dims[0] * S.dims[1] * S.dims[2];
   float *dm = nullptr, *dv = nullptr, *df = nullptr;
   uint8_t* da = nullptr;
   if (m) MPM_HIP(hipMalloc(&dm, sizeof(float) * cells));
   if (vel) MPM_HIP(hipMalloc(&dv, sizeof(float) * 3 * cells));
   if (force) MPM_HIP(hipMalloc(&df, sizeof(float) * 3 * cells));
   if (active) MPM_HIP(hipMalloc(&da, cells));
-  launch_grid_to_dense(make_args(ctx, 0.0f), dm, dv, df, da, ctx->stream);
+  launch_grid_to_dense(make_args(ctx, 0.0f), S, dm, dv, df, da, ctx->stream);
   if (m) MPM_HIP(hipMemcpyAsync(m, dm, sizeof(float) * cells, hipMemcpyDeviceToHost, ctx->stream));
   if (vel) MPM_HIP(hipMemcpyAsync(vel, dv, sizeof(float) * 3 * cells, hipMemcpyDeviceToHost, ctx->stream));
   if (force) MPM_HIP(hipMemcpyAsync(force, df, sizeof(float) * 3 * cells, hipMemcpyDeviceToHost, ctx->stream));
@@ -696,6 +950,10 @@ int mpm_get_grid(mpm_ctx* ctx, float* m, float* vel, float* force, uint8_t* acti
   (void)hipFree(df);
   (void)hipFree(da);
   return MPM_OK;
+}
+
+int mpm_get_grid(mpm_ctx* ctx, float* m, float* vel, float* force, uint8_t* active) {
+  return mpm_scene_get_grid(ctx, 0, m, vel, force, active);
 }
 
 }  // extern "C"

@@ -21,6 +21,10 @@
 // any offset.  A particle outside the margin takes a global-memory path.
 // The grid pass then turns accumulators into velocities on the active
 // blocks only.
+//
+// Every block-list entry carries its scene: one launch steps all scenes of a
+// context, each with its own grid spacing, grid, material and colliders read
+// from the device scene table.
 #include "mpm_fused.hpp"
 
 namespace mpm {
@@ -31,7 +35,7 @@ constexpr int HN = 9;
 constexpr int SX = 112, SY = 12;
 constexpr int PLANE = 1008;  // >= 8*SX + 8*SY + 8 + 1
 
-enum Mode { MODE_P2G = 0, MODE_FUSED = 1, MODE_G2P = 2 };
+enum Mode { MODE_P2G = 0, MODE_FUSED = 1, MODE_G2P = 2, MODE_MASS = 3 };
 
 // Field f of slot p: a wave-uniform row base plus a 32-bit byte offset, which
 // maps to one scalar-base global access per field (slots < 2^30).
@@ -44,10 +48,9 @@ __device__ __forceinline__ void st(const KArgs& a, int f, int p, float v) {
   *reinterpret_cast<float*>(row + (uint32_t)p * 4u) = v;
 }
 
-__device__ __forceinline__ void mark_block(const BlockArgs& ba, int bi, int bj, int bk) {
-  const KArgs& a = ba.a;
-  if (bi < 0 || bj < 0 || bk < 0 || bi >= a.nb[0] || bj >= a.nb[1] || bk >= a.nb[2]) return;
-  const int gb = (bi * a.nb[1] + bj) * a.nb[2] + bk;
+__device__ __forceinline__ void mark_block(const BlockArgs& ba, const Scene& S, int bi, int bj, int bk) {
+  if (bi < 0 || bj < 0 || bk < 0 || bi >= S.nb[0] || bj >= S.nb[1] || bk >= S.nb[2]) return;
+  const int gb = (int)(S.blk0 + (bi * S.nb[1] + bj) * S.nb[2] + bk);
   const int step = *ba.dstep;
   if (atomicExch(&ba.gflag[gb], step) != step) {
     const int slot = atomicAdd(ba.gcnt + step % 3, 1);
@@ -122,11 +125,11 @@ struct StencilInfo {
   int lb[3];   // stencil origin inside the halo
   bool fits;   // stencil inside the halo
 };
-__device__ __forceinline__ StencilInfo stencil_at(const KArgs& a, const int o[3], F3 x) {
+__device__ __forceinline__ StencilInfo stencil_at(const Scene& S, const int o[3], F3 x) {
   StencilInfo s;
-  s.ax[0] = axis_weights(x.x, a.h, a.inv_h, 1, a.dims[0] - 3);
-  s.ax[1] = axis_weights(x.y, a.h, a.inv_h, 1, a.dims[1] - 3);
-  s.ax[2] = axis_weights(x.z, a.h, a.inv_h, 1, a.dims[2] - 3);
+  s.ax[0] = axis_weights(x.x, S.h, S.inv_h, 1, S.dims[0] - 3);
+  s.ax[1] = axis_weights(x.y, S.h, S.inv_h, 1, S.dims[1] - 3);
+  s.ax[2] = axis_weights(x.z, S.h, S.inv_h, 1, S.dims[2] - 3);
   s.fits = true;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -138,16 +141,15 @@ __device__ __forceinline__ StencilInfo stencil_at(const KArgs& a, const int o[3]
 
 // G2P of one particle from the LDS halo, or from global memory when its
 // stencil left the halo.
-__device__ __forceinline__ void g2p_one(const BlockArgs& ba, const float* __restrict__ vh, const int o[3], F3 x,
-                                        F3& v, M3& B) {
-  const KArgs& a = ba.a;
-  const StencilInfo s = stencil_at(a, o, x);
+__device__ __forceinline__ void g2p_one(const KArgs& a, const Scene& S, const float* __restrict__ vh, const int o[3],
+                                        F3 x, F3& v, M3& B) {
+  const StencilInfo s = stencil_at(S, o, x);
   float dd[3][4];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const float xc = d == 0 ? x.x : (d == 1 ? x.y : x.z);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) dd[d][k] = (float)(s.ax[d].base - 1 + k) * a.h - xc;
+    for (int k = 0; k < 4; ++k) dd[d][k] = (float)(s.ax[d].base - 1 + k) * S.h - xc;
   }
   if (s.fits) {
     const int q0 = s.lb[0] * SX + s.lb[1] * SY + s.lb[2];
@@ -157,7 +159,7 @@ __device__ __forceinline__ void g2p_one(const BlockArgs& ba, const float* __rest
     }, v, B);
   } else {
     g2p_accum(s.ax, dd, [&](int ia, int ib, int ic) {
-      const float4 g = a.gvel[node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, a.nb)];
+      const float4 g = a.gvel[node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, S)];
       return f3(g.x, g.y, g.z);
     }, v, B);
   }
@@ -166,23 +168,23 @@ __device__ __forceinline__ void g2p_one(const BlockArgs& ba, const float* __rest
 // P2G scatter of mass and of m v + Q (x_i - x_p) over the 4x4x4 stencil
 // (cpp:99-117 and cpp:249-252 combined); `add(ia, ib, ic, m, px, py, pz)`.
 template <typename Add>
-__device__ __forceinline__ void p2g_scatter(const KArgs& a, const StencilInfo& s, F3 x, float m, F3 mv, const M3& Q,
+__device__ __forceinline__ void p2g_scatter(const Scene& S, const StencilInfo& s, F3 x, float m, F3 mv, const M3& Q,
                                             Add add) {
 #pragma unroll
   for (int ia = 0; ia < 4; ++ia) {
-    const float ddx = (float)(s.ax[0].base - 1 + ia) * a.h - x.x;
+    const float ddx = (float)(s.ax[0].base - 1 + ia) * S.h - x.x;
     const float wa = s.ax[0].w[ia];
     const float qax = fmaf(Q.c[0][0], ddx, mv.x), qay = fmaf(Q.c[0][1], ddx, mv.y), qaz = fmaf(Q.c[0][2], ddx, mv.z);
 #pragma unroll
     for (int ib = 0; ib < 4; ++ib) {
-      const float ddy = (float)(s.ax[1].base - 1 + ib) * a.h - x.y;
+      const float ddy = (float)(s.ax[1].base - 1 + ib) * S.h - x.y;
       const float wxy = wa * s.ax[1].w[ib];
       const float qx = fmaf(Q.c[1][0], ddy, qax);
       const float qy = fmaf(Q.c[1][1], ddy, qay);
       const float qz = fmaf(Q.c[1][2], ddy, qaz);
 #pragma unroll
       for (int ic = 0; ic < 4; ++ic) {
-        const float ddz = (float)(s.ax[2].base - 1 + ic) * a.h - x.z;
+        const float ddz = (float)(s.ax[2].base - 1 + ic) * S.h - x.z;
         const float w = wxy * s.ax[2].w[ic];
         add(ia, ib, ic, m * w, w * fmaf(Q.c[2][0], ddz, qx), w * fmaf(Q.c[2][1], ddz, qy),
             w * fmaf(Q.c[2][2], ddz, qz));
@@ -191,10 +193,43 @@ __device__ __forceinline__ void p2g_scatter(const KArgs& a, const StencilInfo& s
   }
 }
 
+// Scatter into the LDS halo, or with global atomics (and the touched grid
+// blocks marked) when the stencil left the halo.
+__device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S, double* __restrict__ ah,
+                                            const int o[3], F3 x, float m, F3 mv, const M3& Q) {
+  const KArgs& a = ba.a;
+  const StencilInfo s = stencil_at(S, o, x);
+  if (s.fits) {
+    const int q0 = s.lb[0] * SX + s.lb[1] * SY + s.lb[2];
+    p2g_scatter(S, s, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
+      const int q = q0 + ia * SX + ib * SY + ic;
+      atomicAdd(&ah[q], (double)vm);
+      atomicAdd(&ah[PLANE + q], (double)px);
+      atomicAdd(&ah[2 * PLANE + q], (double)py);
+      atomicAdd(&ah[3 * PLANE + q], (double)pz);
+    });
+  } else {
+    // stencil blocks: base-1 .. base+2 spans at most two blocks per axis
+    for (int di = 0; di < 2; ++di)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int dk = 0; dk < 2; ++dk)
+          mark_block(ba, S, (s.ax[0].base - 1 + 3 * di) >> 2, (s.ax[1].base - 1 + 3 * dj) >> 2,
+                     (s.ax[2].base - 1 + 3 * dk) >> 2);
+    p2g_scatter(S, s, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
+      float* g = reinterpret_cast<float*>(
+          a.gacc + node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, S));
+      atomicAdd(g + 0, vm);
+      atomicAdd(g + 1, px);
+      atomicAdd(g + 2, py);
+      atomicAdd(g + 3, pz);
+    });
+  }
+}
+
 // Stress from F^n, F update F^n -> F^{n+1} with B, then the P2G scatter.
 template <int COMPOSE>
-__device__ __forceinline__ void p2g_one(const BlockArgs& ba, double* __restrict__ ah, const int o[3], int p, F3 x,
-                                        F3 v, const M3& B) {
+__device__ __forceinline__ void p2g_one(const BlockArgs& ba, const Scene& S, int scene, double* __restrict__ ah,
+                                        const int o[3], int p, F3 x, F3 v, const M3& B) {
   const KArgs& a = ba.a;
   M3 Q;
   float m;
@@ -210,13 +245,14 @@ __device__ __forceinline__ void p2g_one(const BlockArgs& ba, double* __restrict_
     m = ld(a, FM, p);
     const float V0 = ld(a, FVOL, p);
     const float E = ld(a, FE0, p);
-    const M3 A = stress_matrix(E, a.lame, V0, a.Dinv, FE, FP);
-    const M3 BD = mmul(B, a.Dinv);
+    const M3 Dinv = dinv_matrix(S.dinv);
+    const M3 A = stress_matrix(E, S.lame, V0, Dinv, FE, FP);
+    const M3 BD = mmul(B, Dinv);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int r = 0; r < 3; ++r) Q.c[c][r] = fmaf(m, BD.c[c][r], -a.dt * A.c[c][r]);
-    if (f_update_t<COMPOSE>(a.dt, a.Dinv, a.lo, a.hi, a.gap64, B, FE, FP)) {
+    if (f_update_t<COMPOSE>(a.dt, Dinv, S.lo, S.hi, S.gap64, B, FE, FP)) {
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -225,49 +261,29 @@ __device__ __forceinline__ void p2g_one(const BlockArgs& ba, double* __restrict_
           st(a, FFP + 3 * c + r, p, FP.c[c][r]);
         }
     } else {
-      atomicOr(a.status, ST_NONFINITE);
+      atomicOr(a.status + scene, ST_NONFINITE);
     }
   }
-  const F3 mv = v * m;
-  const StencilInfo s = stencil_at(a, o, x);
-  if (s.fits) {
-    const int q0 = s.lb[0] * SX + s.lb[1] * SY + s.lb[2];
-    p2g_scatter(a, s, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
-      const int q = q0 + ia * SX + ib * SY + ic;
-      atomicAdd(&ah[q], (double)vm);
-      atomicAdd(&ah[PLANE + q], (double)px);
-      atomicAdd(&ah[2 * PLANE + q], (double)py);
-      atomicAdd(&ah[3 * PLANE + q], (double)pz);
-    });
-  } else {
-    // stencil blocks: base-1 .. base+2 spans at most two blocks per axis
-    for (int di = 0; di < 2; ++di)
-      for (int dj = 0; dj < 2; ++dj)
-        for (int dk = 0; dk < 2; ++dk)
-          mark_block(ba, (s.ax[0].base - 1 + 3 * di) >> 2, (s.ax[1].base - 1 + 3 * dj) >> 2,
-                     (s.ax[2].base - 1 + 3 * dk) >> 2);
-    p2g_scatter(a, s, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
-      float* g = reinterpret_cast<float*>(
-          a.gacc + node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, a.nb));
-      atomicAdd(g + 0, vm);
-      atomicAdd(g + 1, px);
-      atomicAdd(g + 2, py);
-      atomicAdd(g + 3, pz);
-    });
-  }
+  scatter_one(ba, S, ah, o, x, m, v * m, Q);
 }
 
-__device__ __forceinline__ F3 advect(const KArgs& a, F3 x, F3 v) {
+__device__ __forceinline__ F3 advect(const KArgs& a, const Scene& S, F3 x, F3 v) {
   x = x + v * a.dt;
-  const float lo = 3.0f * a.h;
-  x.x = fminf(fmaxf(x.x, lo), (float)(a.dims[0] - 3) * a.h);
-  x.y = fminf(fmaxf(x.y, lo), (float)(a.dims[1] - 3) * a.h);
-  x.z = fminf(fmaxf(x.z, lo), (float)(a.dims[2] - 3) * a.h);
+  const float lo = 3.0f * S.h;
+  x.x = fminf(fmaxf(x.x, lo), (float)(S.dims[0] - 3) * S.h);
+  x.y = fminf(fmaxf(x.y, lo), (float)(S.dims[1] - 3) * S.h);
+  x.z = fminf(fmaxf(x.z, lo), (float)(S.dims[2] - 3) * S.h);
   return x;
 }
 
-template <int MODE, int COMPOSE> __global__ void __launch_bounds__(TPB) k_block_pass(BlockArgs ba) {
-  constexpr bool kG2P = MODE != MODE_P2G;
+// BATCHED = 0 (one scene): the scene comes from the kernel arguments,
+// read-only and loop-invariant; BATCHED = 1: each block's scene from the
+// device table, copied once per block.
+// LITERAL kernels are held to three waves per SIMD (<= 168 registers); the
+// TEXTBOOK ones, with their FP64 re-solve, would spill heavily and keep two.
+template <int MODE, int COMPOSE, int BATCHED>
+__global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 1) k_block_pass(BlockArgs ba) {
+  constexpr bool kG2P = MODE == MODE_FUSED || MODE == MODE_G2P;
   constexpr bool kP2G = MODE != MODE_G2P;
   __shared__ float vh[kG2P ? 3 * PLANE : 1];
   __shared__ double ah[kP2G ? 4 * PLANE : 1];
@@ -276,19 +292,21 @@ template <int MODE, int COMPOSE> __global__ void __launch_bounds__(TPB) k_block_
   const int nblocks = *ba.nblk;
   for (int s = blockIdx.x; s < nblocks; s += gridDim.x) {
     const int4 e = ba.blist[s];
-    const int blk = e.x, start = e.y, end = e.z;
-    const int bk = blk % a.nb[2];
-    const int tt = blk / a.nb[2];
-    const int bj = tt % a.nb[1];
-    const int bi = tt / a.nb[1];
+    const int scene = e.w, start = e.y, end = e.z;
+    const Scene S = BATCHED ? a.scenes[scene] : a.sc;  // by value: particle stores cannot alias it
+    const int blk = (int)(e.x - S.blk0);
+    const int bk = blk % S.nb[2];
+    const int tt = blk / S.nb[2];
+    const int bj = tt % S.nb[1];
+    const int bi = tt / S.nb[1];
     const int o[3] = {4 * bi - 2, 4 * bj - 2, 4 * bk - 2};
     if (kG2P) {
       for (int q = t; q < HN * HN * HN; q += TPB) {
         const int hx = q / (HN * HN), hy = (q / HN) % HN, hz = q % HN;
         const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
         float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (gi >= 0 && gj >= 0 && gk >= 0 && gi < a.dims[0] && gj < a.dims[1] && gk < a.dims[2])
-          g = a.gvel[node_index(gi, gj, gk, a.nb)];
+        if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2])
+          g = a.gvel[node_index(gi, gj, gk, S)];
         const int l = hx * SX + hy * SY + hz;
         vh[l] = g.x;
         vh[PLANE + l] = g.y;
@@ -301,11 +319,16 @@ template <int MODE, int COMPOSE> __global__ void __launch_bounds__(TPB) k_block_
     __syncthreads();
     for (int p = start + t; p < end; p += TPB) {
       F3 x = f3(ld(a, FX, p), ld(a, FX + 1, p), ld(a, FX + 2, p));
+      if (MODE == MODE_MASS) {
+        M3 Z = mdiag(0.0f);
+        scatter_one(ba, S, ah, o, x, ld(a, FM, p), f3(0.f, 0.f, 0.f), Z);
+        continue;
+      }
       F3 v;
       M3 B;
       if (kG2P) {
-        g2p_one(ba, vh, o, x, v, B);
-        x = advect(a, x, v);
+        g2p_one(a, S, vh, o, x, v, B);
+        x = advect(a, S, x, v);
         st(a, FX, p, x.x);
         st(a, FX + 1, p, x.y);
         st(a, FX + 2, p, x.z);
@@ -325,7 +348,7 @@ template <int MODE, int COMPOSE> __global__ void __launch_bounds__(TPB) k_block_
 #pragma unroll
           for (int r = 0; r < 3; ++r) B.c[c][r] = ld(a, FB + 3 * c + r, p);
       }
-      if (kP2G) p2g_one<COMPOSE>(ba, ah, o, p, x, v, B);
+      if (kP2G) p2g_one<COMPOSE>(ba, S, scene, ah, o, p, x, v, B);
     }
     __syncthreads();
     if (kP2G) {
@@ -335,8 +358,8 @@ template <int MODE, int COMPOSE> __global__ void __launch_bounds__(TPB) k_block_
         const float m = (float)ah[l], px = (float)ah[PLANE + l], py = (float)ah[2 * PLANE + l], pz = (float)ah[3 * PLANE + l];
         if (m != 0.0f || px != 0.0f || py != 0.0f || pz != 0.0f) {
           const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
-          if (gi >= 0 && gj >= 0 && gk >= 0 && gi < a.dims[0] && gj < a.dims[1] && gk < a.dims[2]) {
-            float* g = reinterpret_cast<float*>(a.gacc + node_index(gi, gj, gk, a.nb));
+          if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2]) {
+            float* g = reinterpret_cast<float*>(a.gacc + node_index(gi, gj, gk, S));
             atomicAdd(g + 0, m);
             atomicAdd(g + 1, px);
             atomicAdd(g + 2, py);
@@ -344,7 +367,7 @@ template <int MODE, int COMPOSE> __global__ void __launch_bounds__(TPB) k_block_
           }
         }
       }
-      if (t < 27) mark_block(ba, bi - 1 + t / 9, bj - 1 + (t / 3) % 3, bk - 1 + t % 3);
+      if (t < 27) mark_block(ba, S, bi - 1 + t / 9, bj - 1 + (t / 3) % 3, bk - 1 + t % 3);
     }
     __syncthreads();
   }
@@ -352,7 +375,7 @@ template <int MODE, int COMPOSE> __global__ void __launch_bounds__(TPB) k_block_
 
 // v = p / m + dt g on the nodes of active blocks, boundary conditions, and
 // the accumulator reset; blocks active last substep but not now are zeroed.
-__global__ void __launch_bounds__(TPB) k_grid_active(BlockArgs ba) {
+template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(BlockArgs ba) {
   const KArgs& a = ba.a;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -368,16 +391,18 @@ __global__ void __launch_bounds__(TPB) k_grid_active(BlockArgs ba) {
       const float4 acc = a.gacc[g];
       float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
       if (acc.x != 0.0f) {
-        F3 v = f3(acc.y / acc.x + a.dt * a.g[0], acc.z / acc.x + a.dt * a.g[1], acc.w / acc.x + a.dt * a.g[2]);
-        if (a.bc_mode) {
-          const int bk = gb % a.nb[2];
-          const int t2 = gb / a.nb[2];
-          const int bj = t2 % a.nb[1];
-          const int bi = t2 / a.nb[1];
+        const Scene S = BATCHED ? a.scenes[a.blk_scene[gb]] : a.sc;
+        F3 v = f3(acc.y / acc.x + a.dt * S.g[0], acc.z / acc.x + a.dt * S.g[1], acc.w / acc.x + a.dt * S.g[2]);
+        if (S.bc_mode) {
+          const int lb = (int)(gb - S.blk0);
+          const int bk = lb % S.nb[2];
+          const int t2 = lb / S.nb[2];
+          const int bj = t2 % S.nb[1];
+          const int bi = t2 / S.nb[1];
           const int i = bi * 4 + (lane >> 4), j = bj * 4 + ((lane >> 2) & 3), k = bk * 4 + (lane & 3);
-          if ((a.bc_mode & MPM_BC_COLLIDERS) && a.ncoll > 0)
-            v = body_collision(a.coll, a.ncoll, f3((float)i * a.h, (float)j * a.h, (float)k * a.h), v);
-          if (a.bc_mode & MPM_BC_DOMAIN) v = domain_walls(i, j, k, a.dims[0], a.dims[1], a.dims[2], v);
+          if ((S.bc_mode & MPM_BC_COLLIDERS) && S.ncoll > 0)
+            v = body_collision(S.coll, S.ncoll, f3((float)i * S.h, (float)j * S.h, (float)k * S.h), v);
+          if (S.bc_mode & MPM_BC_DOMAIN) v = domain_walls(i, j, k, S.dims[0], S.dims[1], S.dims[2], v);
         }
         out = make_float4(v.x, v.y, v.z, acc.x);
       }
@@ -403,35 +428,40 @@ __global__ void k_advance_step(int* dstep, int* gcnt) {
 
 void launch_advance_step(int* dstep, int* gcnt, hipStream_t s) { k_advance_step<<<1, 1, 0, s>>>(dstep, gcnt); }
 
-static int pass_grid(const BlockArgs& ba) {
-  const int64_t nb = (int64_t)ba.a.nb[0] * ba.a.nb[1] * ba.a.nb[2];
-  return (int)std::max<int64_t>(1, std::min<int64_t>(nb, 2048));
-}
-
-void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s) {
-  const int grid = pass_grid(ba);
+template <int BATCHED> void launch_block_pass_t(const BlockArgs& ba, int mode, hipStream_t s) {
+  const int grid = ba.grid;
   const bool lit = ba.a.compose == COMPOSE_LITERAL;
   switch (mode) {
     case MODE_P2G:
       if (lit)
-        k_block_pass<MODE_P2G, COMPOSE_LITERAL><<<grid, TPB, 0, s>>>(ba);
+        k_block_pass<MODE_P2G, COMPOSE_LITERAL, BATCHED><<<grid, TPB, 0, s>>>(ba);
       else
-        k_block_pass<MODE_P2G, COMPOSE_TEXTBOOK><<<grid, TPB, 0, s>>>(ba);
+        k_block_pass<MODE_P2G, COMPOSE_TEXTBOOK, BATCHED><<<grid, TPB, 0, s>>>(ba);
       break;
     case MODE_FUSED:
       if (lit)
-        k_block_pass<MODE_FUSED, COMPOSE_LITERAL><<<grid, TPB, 0, s>>>(ba);
+        k_block_pass<MODE_FUSED, COMPOSE_LITERAL, BATCHED><<<grid, TPB, 0, s>>>(ba);
       else
-        k_block_pass<MODE_FUSED, COMPOSE_TEXTBOOK><<<grid, TPB, 0, s>>>(ba);
+        k_block_pass<MODE_FUSED, COMPOSE_TEXTBOOK, BATCHED><<<grid, TPB, 0, s>>>(ba);
       break;
-    default: k_block_pass<MODE_G2P, COMPOSE_LITERAL><<<grid, TPB, 0, s>>>(ba); break;
+    case MODE_MASS: k_block_pass<MODE_MASS, COMPOSE_LITERAL, BATCHED><<<grid, TPB, 0, s>>>(ba); break;
+    default: k_block_pass<MODE_G2P, COMPOSE_LITERAL, BATCHED><<<grid, TPB, 0, s>>>(ba); break;
   }
 }
 
+void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s) {
+  if (ba.a.nscenes == 1)
+    launch_block_pass_t<0>(ba, mode, s);
+  else
+    launch_block_pass_t<1>(ba, mode, s);
+}
+
 void launch_grid_active(const BlockArgs& ba, hipStream_t s) {
-  const int64_t nb = (int64_t)ba.a.nb[0] * ba.a.nb[1] * ba.a.nb[2];
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((nb + 3) / 4, 2048));
-  k_grid_active<<<grid, TPB, 0, s>>>(ba);
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((ba.nblocks + 3) / 4, 2048));
+  if (ba.a.nscenes == 1)
+    k_grid_active<0><<<grid, TPB, 0, s>>>(ba);
+  else
+    k_grid_active<1><<<grid, TPB, 0, s>>>(ba);
 }
 
 }  // namespace mpm

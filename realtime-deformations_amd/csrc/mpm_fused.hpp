@@ -16,9 +16,11 @@ struct BlockArgs {
   int* glist[2];      // grid blocks activated, by epoch parity
   int* gcnt;          // [3] counters, by epoch modulo 3
   const int* dstep;   // substep epoch (> 0)
+  int64_t nblocks;    // grid blocks over all scenes
+  int grid;           // workgroups of the persistent block pass
 };
 
-enum { PASS_P2G = 0, PASS_FUSED = 1, PASS_G2P = 2 };
+enum { PASS_P2G = 0, PASS_FUSED = 1, PASS_G2P = 2, PASS_MASS = 3 };
 void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s);
 void launch_grid_active(const BlockArgs& ba, hipStream_t s);
 void launch_advance_step(int* dstep, int* gcnt, hipStream_t s);

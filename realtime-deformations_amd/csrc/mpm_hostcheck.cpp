@@ -23,6 +23,14 @@ M3 dp_inverse(float h) { return minverse(mscale(mscale(mscale(mdiag(1.0f), 1.0f 
 
 extern "C" {
 
+// 1 when glm's inverse of diag(h^2/3) (the operation order of the context's
+// set-up) equals dinv_matrix of its diagonal bit for bit, zero signs included
+int hc_dinv_pattern(float h) {
+  const M3 full = dp_inverse(h);
+  const M3 fast = dinv_matrix(full.c[0][0]);
+  return std::memcmp(&full, &fast, sizeof(M3)) == 0 ? 1 : 0;
+}
+
 // base and the four 1-D weights of every coordinate x[i]
 void hc_axis_weights(const float* x, int n, float h, int lo, int hi, int* base, float* w) {
   const float inv_h = 1.0f / h;

@@ -31,42 +31,73 @@ enum Field : int {
 
 enum StatusBits : int { ST_NONFINITE = 1, ST_BOUNDS = 2 };
 
+// One scene: its grid (dims, spacing, place in the shared grid arrays), its
+// particle slots, material and boundary conditions.  A context steps any
+// number of scenes together; kernels read a scene's entry per block or per
+// particle from a device table.
+struct Scene {
+  int dims[3];
+  int nb[3];        // 4x4x4 node blocks per axis
+  int64_t blk0;     // first grid block of this scene in the shared grid arrays
+  int64_t nblocks;  // nb0 * nb1 * nb2
+  int p0, np;       // particle slots [p0, p0 + np) (fixed: sorting stays inside a scene)
+  float h, inv_h;   // inv_h: 1/h correctly rounded
+  float E;          // Young's modulus of particles uploaded without E0
+  float lo, hi, gap64;
+  float dinv;       // D^-1 = dinv_matrix(dinv), glm inverse(mat3(1) h^2 / 3)
+  Lame lame;
+  float g[3];
+  int bc_mode, ncoll;
+  const DevCollider* coll;
+};
+
 // Everything a kernel needs, passed by value.
 struct KArgs {
   float* P;
   int64_t stride;
-  int n;
+  int n;           // particles over all scenes
   float4* gacc;    // (m, p) accumulators
   float4* gvel;    // (v, m) after the grid update
   float4* gforce;  // (f, 0), legacy stage path only
-  int dims[3];
-  int nb[3];
-  int64_t nnodes;  // nb0*nb1*nb2*64
-  float h, inv_h, dt, lo, hi, gap64;  // inv_h: 1/h correctly rounded
-  M3 Dinv;
-  Lame lame;
-  float g[3];
-  int compose, bc_mode, ncoll;
-  const DevCollider* coll;
-  int* status;
+  int64_t nnodes;  // nodes over all scenes
+  float dt;
+  int compose;
+  int nscenes;
+  const Scene* scenes;   // device table
+  const int* blk_scene;  // grid block -> scene
+  Scene sc;              // scene 0 by value (legacy stage kernels: one scene)
+  int* status;           // per scene
 };
 
-__device__ __forceinline__ int64_t node_index(int i, int j, int k, const int nb[3]) {
-  return ((int64_t)((i >> 2) * nb[1] + (j >> 2)) * nb[2] + (k >> 2)) * 64 + ((i & 3) << 4) + ((j & 3) << 2) +
-         (k & 3);
+__device__ __forceinline__ int64_t node_index(int i, int j, int k, const Scene& S) {
+  return (S.blk0 + ((int64_t)((i >> 2) * S.nb[1] + (j >> 2)) * S.nb[2] + (k >> 2))) * 64 + ((i & 3) << 4) +
+         ((j & 3) << 2) + (k & 3);
+}
+
+// scene owning particle slot p (slots are grouped by scene)
+__device__ __forceinline__ int scene_of_slot(const Scene* scenes, int nscenes, int p) {
+  int lo = 0, hi = nscenes - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (scenes[mid].p0 <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
 }
 
 }  // namespace mpm
 
 struct mpm_ctx {
-  mpm_params prm{};
+  mpm_params prm{};            // scene 0's parameters (compose, resort_every for all)
+  int nscenes = 1;
+  std::vector<mpm::Scene> scenes;          // host copy of the scene table
+  std::vector<mpm::DevCollider*> scoll;    // per-scene collider arrays (device)
+  mpm::Scene* d_scenes = nullptr;          // device scene table
+  int* blk_scene = nullptr;                // grid block -> scene
   int n = 0;
   int64_t stride = 0;
-  int dims[3] = {0, 0, 0};
-  int nb[3] = {0, 0, 0};
-  int64_t nnodes = 0;
-  float h = 0, lo = 0, hi = 0;
-  mpm::M3 Dinv{};
+  int64_t nnodes = 0;       // over all scenes
+  int64_t nblocks = 0;      // grid blocks over all scenes
   float* P = nullptr;       // particle fields (binned order once sorted)
   float* P2 = nullptr;      // reorder target, swapped with P
   int* pid = nullptr;       // caller index of each particle slot
@@ -82,23 +113,23 @@ struct mpm_ctx {
   int* dstep = nullptr;     // device substep epoch (> 0)
   int since_sort = -1;      // substeps since the last binning (-1: never binned)
   int resort_every = 16;
+  int block_grid = 2048;    // workgroups of the persistent block pass
   float4* gacc = nullptr;
   float4* gvel = nullptr;
   float4* gforce = nullptr;
-  mpm::DevCollider* coll = nullptr;
-  int ncoll = 0;
-  int* status = nullptr;  // device status word
+  int* status = nullptr;    // device status word per scene
   hipStream_t stream = nullptr;
-  bool particles_set = false;
-  bool gacc_clean = false;  // fused path needs zeroed accumulators
+  std::vector<char> scene_set;  // particles uploaded, per scene
+  bool particles_set = false;   // every scene has particles
+  bool gacc_clean = false;      // fused path needs zeroed accumulators
   std::string err;
   // HIP graphs of [rebin + resort_every substeps], one per particle-buffer
-  // parity, valid for one dt and collider table
+  // parity, valid for one dt (scene data, colliders included, is read from
+  // the device table at run time)
   float* P_first = nullptr;  // P as allocated: parity 0 when P == P_first
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   float graph_dt[2] = {0.f, 0.f};
-  const void* graph_coll[2] = {nullptr, nullptr};
-  int graph_ncoll[2] = {-1, -1};
+  int graph_key[2] = {0, 0};  // resort_every and block_grid baked into the graph
   bool use_graphs = true;
   // event profiling of fused-path launches
   bool profiling = false;
@@ -124,13 +155,18 @@ void launch_stage_fupdate(const KArgs& a, hipStream_t s);
 void launch_stage_g2p(const KArgs& a, hipStream_t s);
 void launch_stage_advect(const KArgs& a, hipStream_t s);
 // layout conversion (pid: caller index of each slot)
-void launch_scatter_field(float* dst, int64_t stride, const float* src_aos, const int* pid, int n, int comps,
-                          hipStream_t s);
-void launch_gather_field(float* dst_aos, const float* src, int64_t stride, const int* pid, int n, int comps,
+// slots [p0, p0 + np) of one scene; pid holds caller indices, src/dst are
+// the scene's caller-ordered arrays (index pid - p0)
+void launch_scatter_field(float* dst, int64_t stride, const float* src_aos, const int* pid, int p0, int np,
+                          int comps, hipStream_t s);
+void launch_gather_field(float* dst_aos, const float* src, int64_t stride, const int* pid, int p0, int np, int comps,
                          hipStream_t s);
 void launch_iota(int* p, int n, hipStream_t s);
 void launch_fill_int(int* p, int64_t n, int v, hipStream_t s);
-void launch_grid_to_dense(const KArgs& a, float* m, float* vel, float* force, uint8_t* active, hipStream_t s);
+void launch_grid_to_dense(const KArgs& a, const Scene& S, float* m, float* vel, float* force, uint8_t* active,
+                         hipStream_t s);
 void host_collider_inverse(const float t[3], const float q[4], float inv[4][4]);
 void launch_count_active(const KArgs& a, int64_t* out, hipStream_t s);
+// batched initial volumes: density gathered from the node masses in gvel.w
+void launch_volumes(const KArgs& a, hipStream_t s);
 }  // namespace mpm

@@ -35,11 +35,11 @@ __device__ __forceinline__ void store_f3(float* P, int64_t stride, int f, int p,
 struct Stencil {
   Axis ax[3];
 };
-__device__ __forceinline__ Stencil stencil_of(const KArgs& a, F3 x) {
+__device__ __forceinline__ Stencil stencil_of(const Scene& S, F3 x) {
   Stencil s;
-  s.ax[0] = axis_weights(x.x, a.h, a.inv_h, 1, a.dims[0] - 3);
-  s.ax[1] = axis_weights(x.y, a.h, a.inv_h, 1, a.dims[1] - 3);
-  s.ax[2] = axis_weights(x.z, a.h, a.inv_h, 1, a.dims[2] - 3);
+  s.ax[0] = axis_weights(x.x, S.h, S.inv_h, 1, S.dims[0] - 3);
+  s.ax[1] = axis_weights(x.y, S.h, S.inv_h, 1, S.dims[1] - 3);
+  s.ax[2] = axis_weights(x.z, S.h, S.inv_h, 1, S.dims[2] - 3);
   return s;
 }
 
@@ -61,17 +61,17 @@ __global__ void k_stage_p2g(KArgs a) {
   M3 B;
   load_m3(a.P, a.stride, FB, p, B);
   const float m = a.P[(int64_t)FM * a.stride + p];
-  const M3 BD = mmul(B, a.Dinv);
-  const Stencil s = stencil_of(a, x);
+  const M3 BD = mmul(B, dinv_matrix(a.sc.dinv));
+  const Stencil s = stencil_of(a.sc, x);
   for (int dx = 0; dx < 4; ++dx)
     for (int dy = 0; dy < 4; ++dy)
 #pragma unroll
       for (int dz = 0; dz < 4; ++dz) {
         const int i = s.ax[0].base - 1 + dx, j = s.ax[1].base - 1 + dy, k = s.ax[2].base - 1 + dz;
         const float w = s.ax[0].w[dx] * s.ax[1].w[dy] * s.ax[2].w[dz];
-        const F3 Xi = f3((float)i * a.h, (float)j * a.h, (float)k * a.h);
+        const F3 Xi = f3((float)i * a.sc.h, (float)j * a.sc.h, (float)k * a.sc.h);
         const F3 mom = (v + mvec(BD, Xi - x)) * (w * m);
-        float* g = reinterpret_cast<float*>(a.gacc + node_index(i, j, k, a.nb));
+        float* g = reinterpret_cast<float*>(a.gacc + node_index(i, j, k, a.sc));
         atomicAdd(g + 0, m * w);
         atomicAdd(g + 1, mom.x);
         atomicAdd(g + 2, mom.y);
@@ -79,17 +79,18 @@ __global__ void k_stage_p2g(KArgs a) {
       }
 }
 
-__device__ __forceinline__ bool node_coords(const KArgs& a, int64_t g, int& i, int& j, int& k) {
+// coordinates of node g (scene-local index) of scene S
+__device__ __forceinline__ bool node_coords(const Scene& S, int64_t g, int& i, int& j, int& k) {
   const int64_t blk = g >> 6;
   const int l = (int)(g & 63);
-  const int bk = (int)(blk % a.nb[2]);
-  const int64_t t = blk / a.nb[2];
-  const int bj = (int)(t % a.nb[1]);
-  const int bi = (int)(t / a.nb[1]);
+  const int bk = (int)(blk % S.nb[2]);
+  const int64_t t = blk / S.nb[2];
+  const int bj = (int)(t % S.nb[1]);
+  const int bi = (int)(t / S.nb[1]);
   i = bi * 4 + (l >> 4);
   j = bj * 4 + ((l >> 2) & 3);
   k = bk * 4 + (l & 3);
-  return i < a.dims[0] && j < a.dims[1] && k < a.dims[2];
+  return i < S.dims[0] && j < S.dims[1] && k < S.dims[2];
 }
 
 // velocity = momentum / mass on nodes with mass (cpp:118-121)
@@ -107,7 +108,7 @@ __global__ void k_stage_volumes(KArgs a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
   const F3 x = load_f3(a.P, a.stride, FX, p);
-  const Stencil s = stencil_of(a, x);
+  const Stencil s = stencil_of(a.sc, x);
   float density = 0.0f;
   for (int dx = 0; dx < 4; ++dx)
     for (int dy = 0; dy < 4; ++dy)
@@ -115,9 +116,9 @@ __global__ void k_stage_volumes(KArgs a) {
       for (int dz = 0; dz < 4; ++dz) {
         const int i = s.ax[0].base - 1 + dx, j = s.ax[1].base - 1 + dy, k = s.ax[2].base - 1 + dz;
         const float w = s.ax[0].w[dx] * s.ax[1].w[dy] * s.ax[2].w[dz];
-        density += a.gvel[node_index(i, j, k, a.nb)].w * w;
+        density += a.gvel[node_index(i, j, k, a.sc)].w * w;
       }
-  density /= (a.h * a.h * a.h);
+  density /= (a.sc.h * a.sc.h * a.sc.h);
   const float m = a.P[(int64_t)FM * a.stride + p];
   a.P[(int64_t)FVOL * a.stride + p] = density != 0.0f ? (m / density) : 0.0f;
 }
@@ -132,17 +133,17 @@ __global__ void k_stage_forces(KArgs a) {
   load_m3(a.P, a.stride, FFP, p, FP);
   const float V0 = a.P[(int64_t)FVOL * a.stride + p];
   const float E = a.P[(int64_t)FE0 * a.stride + p];
-  const M3 A = stress_matrix(E, a.lame, V0, a.Dinv, FE, FP);
-  const Stencil s = stencil_of(a, x);
+  const M3 A = stress_matrix(E, a.sc.lame, V0, dinv_matrix(a.sc.dinv), FE, FP);
+  const Stencil s = stencil_of(a.sc, x);
   for (int dx = 0; dx < 4; ++dx)
     for (int dy = 0; dy < 4; ++dy)
 #pragma unroll
       for (int dz = 0; dz < 4; ++dz) {
         const int i = s.ax[0].base - 1 + dx, j = s.ax[1].base - 1 + dy, k = s.ax[2].base - 1 + dz;
         const float w = s.ax[0].w[dx] * s.ax[1].w[dy] * s.ax[2].w[dz];
-        const F3 Xi = f3((float)i * a.h, (float)j * a.h, (float)k * a.h);
+        const F3 Xi = f3((float)i * a.sc.h, (float)j * a.sc.h, (float)k * a.sc.h);
         const F3 f = mvec(mscale(A, w), Xi - x);
-        float* g = reinterpret_cast<float*>(a.gforce + node_index(i, j, k, a.nb));
+        float* g = reinterpret_cast<float*>(a.gforce + node_index(i, j, k, a.sc));
         atomicAdd(g + 0, -f.x);
         atomicAdd(g + 1, -f.y);
         atomicAdd(g + 2, -f.z);
@@ -156,7 +157,7 @@ __global__ void k_stage_grid(KArgs a) {
   float4 v = a.gvel[g];
   if (v.w == 0.0f) return;
   const float4 f = a.gforce[g];
-  const F3 acc = f3(f.x / v.w + a.g[0], f.y / v.w + a.g[1], f.z / v.w + a.g[2]);
+  const F3 acc = f3(f.x / v.w + a.sc.g[0], f.y / v.w + a.sc.g[1], f.z / v.w + a.sc.g[2]);
   v.x = v.x + a.dt * acc.x;
   v.y = v.y + a.dt * acc.y;
   v.z = v.z + a.dt * acc.z;
@@ -164,9 +165,9 @@ __global__ void k_stage_grid(KArgs a) {
 }
 
 __device__ __forceinline__ F3 apply_bc(const KArgs& a, int i, int j, int k, F3 v) {
-  if ((a.bc_mode & MPM_BC_COLLIDERS) && a.ncoll > 0)
-    v = body_collision(a.coll, a.ncoll, f3((float)i * a.h, (float)j * a.h, (float)k * a.h), v);
-  if (a.bc_mode & MPM_BC_DOMAIN) v = domain_walls(i, j, k, a.dims[0], a.dims[1], a.dims[2], v);
+  if ((a.sc.bc_mode & MPM_BC_COLLIDERS) && a.sc.ncoll > 0)
+    v = body_collision(a.sc.coll, a.sc.ncoll, f3((float)i * a.sc.h, (float)j * a.sc.h, (float)k * a.sc.h), v);
+  if (a.sc.bc_mode & MPM_BC_DOMAIN) v = domain_walls(i, j, k, a.sc.dims[0], a.sc.dims[1], a.sc.dims[2], v);
   return v;
 }
 
@@ -177,7 +178,7 @@ __global__ void k_stage_collide(KArgs a) {
   float4 v = a.gvel[g];
   if (v.w == 0.0f) return;
   int i, j, k;
-  if (!node_coords(a, g, i, j, k)) return;
+  if (!node_coords(a.sc, g, i, j, k)) return;
   const F3 r = apply_bc(a, i, j, k, f3(v.x, v.y, v.z));
   a.gvel[g] = make_float4(r.x, r.y, r.z, v.w);
 }
@@ -190,7 +191,7 @@ __global__ void k_stage_fupdate(KArgs a) {
   load_m3(a.P, a.stride, FB, p, B);
   load_m3(a.P, a.stride, FFE, p, FE);
   load_m3(a.P, a.stride, FFP, p, FP);
-  if (!f_update(a.compose, a.dt, a.Dinv, a.lo, a.hi, a.gap64, B, FE, FP)) {
+  if (!f_update(a.compose, a.dt, dinv_matrix(a.sc.dinv), a.sc.lo, a.sc.hi, a.sc.gap64, B, FE, FP)) {
     atomicOr(a.status, ST_NONFINITE);
     return;
   }
@@ -203,7 +204,7 @@ __global__ void k_stage_g2p(KArgs a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
   const F3 x = load_f3(a.P, a.stride, FX, p);
-  const Stencil s = stencil_of(a, x);
+  const Stencil s = stencil_of(a.sc, x);
   F3 v = f3(0.f, 0.f, 0.f);
   M3 B = mdiag(0.0f);
   for (int dx = 0; dx < 4; ++dx)
@@ -212,10 +213,10 @@ __global__ void k_stage_g2p(KArgs a) {
       for (int dz = 0; dz < 4; ++dz) {
         const int i = s.ax[0].base - 1 + dx, j = s.ax[1].base - 1 + dy, k = s.ax[2].base - 1 + dz;
         const float w = s.ax[0].w[dx] * s.ax[1].w[dy] * s.ax[2].w[dz];
-        const float4 gv4 = a.gvel[node_index(i, j, k, a.nb)];
+        const float4 gv4 = a.gvel[node_index(i, j, k, a.sc)];
         const F3 gv = f3(gv4.x, gv4.y, gv4.z);
         v = v + gv * w;
-        const F3 d = f3((float)i * a.h, (float)j * a.h, (float)k * a.h) - x;
+        const F3 d = f3((float)i * a.sc.h, (float)j * a.sc.h, (float)k * a.sc.h) - x;
         const float dd[3] = {d.x, d.y, d.z};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -235,23 +236,27 @@ __global__ void k_stage_advect(KArgs a) {
   F3 x = load_f3(a.P, a.stride, FX, p);
   const F3 v = load_f3(a.P, a.stride, FV, p);
   x = x + v * a.dt;
-  const float lo = 3.0f * a.h;
-  x.x = fminf(fmaxf(x.x, lo), (float)(a.dims[0] - 3) * a.h);
-  x.y = fminf(fmaxf(x.y, lo), (float)(a.dims[1] - 3) * a.h);
-  x.z = fminf(fmaxf(x.z, lo), (float)(a.dims[2] - 3) * a.h);
+  const float lo = 3.0f * a.sc.h;
+  x.x = fminf(fmaxf(x.x, lo), (float)(a.sc.dims[0] - 3) * a.sc.h);
+  x.y = fminf(fmaxf(x.y, lo), (float)(a.sc.dims[1] - 3) * a.sc.h);
+  x.z = fminf(fmaxf(x.z, lo), (float)(a.sc.dims[2] - 3) * a.sc.h);
   store_f3(a.P, a.stride, FX, p, x);
 }
 
-__global__ void k_scatter_field(float* dst, int64_t stride, const float* src, const int* pid, int n, int comps) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int q = pid ? pid[p] : p;
+__global__ void k_scatter_field(float* dst, int64_t stride, const float* src, const int* pid, int p0, int np,
+                                int comps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int p = p0 + i;
+  const int q = pid[p] - p0;
   for (int c = 0; c < comps; ++c) dst[(int64_t)c * stride + p] = src[(int64_t)q * comps + c];
 }
-__global__ void k_gather_field(float* dst, const float* src, int64_t stride, const int* pid, int n, int comps) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int q = pid ? pid[p] : p;
+__global__ void k_gather_field(float* dst, const float* src, int64_t stride, const int* pid, int p0, int np,
+                               int comps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int p = p0 + i;
+  const int q = pid[p] - p0;
   for (int c = 0; c < comps; ++c) dst[(int64_t)q * comps + c] = src[(int64_t)c * stride + p];
 }
 __global__ void k_iota(int* p, int n) {
@@ -263,13 +268,14 @@ __global__ void k_fill_int(int* p, int64_t n, int v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_grid_to_dense(KArgs a, float* m, float* vel, float* force, uint8_t* active) {
+__global__ void k_grid_to_dense(KArgs a, Scene S, float* m, float* vel, float* force, uint8_t* active) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.nnodes) return;
+  if (g >= S.nblocks * 64) return;
   int i, j, k;
-  if (!node_coords(a, g, i, j, k)) return;
-  const int64_t d = ((int64_t)i * a.dims[1] + j) * a.dims[2] + k;
-  const float4 v = a.gvel[g];
+  if (!node_coords(S, g, i, j, k)) return;
+  const int64_t d = ((int64_t)i * S.dims[1] + j) * S.dims[2] + k;
+  const int64_t gg = S.blk0 * 64 + g;
+  const float4 v = a.gvel[gg];
   if (m) m[d] = v.w;
   if (vel) {
     vel[3 * d] = v.x;
@@ -277,12 +283,34 @@ __global__ void k_grid_to_dense(KArgs a, float* m, float* vel, float* force, uin
     vel[3 * d + 2] = v.z;
   }
   if (force) {
-    const float4 f = a.gforce[g];
+    const float4 f = a.gforce[gg];
     force[3 * d] = f.x;
     force[3 * d + 1] = f.y;
     force[3 * d + 2] = f.z;
   }
   if (active) active[d] = v.w != 0.0f ? 1 : 0;
+}
+
+// computeParticleVolumesAndDensities (cpp:131-142) for every scene: node
+// masses (gvel.w) gathered with the particle's weights, V0 = m / density
+__global__ void k_volumes(KArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const Scene S = a.scenes[scene_of_slot(a.scenes, a.nscenes, p)];
+  const F3 x = load_f3(a.P, a.stride, FX, p);
+  const Stencil s = stencil_of(S, x);
+  float density = 0.0f;
+  for (int dx = 0; dx < 4; ++dx)
+    for (int dy = 0; dy < 4; ++dy)
+#pragma unroll
+      for (int dz = 0; dz < 4; ++dz) {
+        const int i = s.ax[0].base - 1 + dx, j = s.ax[1].base - 1 + dy, k = s.ax[2].base - 1 + dz;
+        const float w = s.ax[0].w[dx] * s.ax[1].w[dy] * s.ax[2].w[dz];
+        density += a.gvel[node_index(i, j, k, S)].w * w;
+      }
+  density /= (S.h * S.h * S.h);
+  const float m = a.P[(int64_t)FM * a.stride + p];
+  a.P[(int64_t)FVOL * a.stride + p] = density != 0.0f ? (m / density) : 0.0f;
 }
 
 __global__ void k_count_active(KArgs a, int64_t* out) {
@@ -321,13 +349,13 @@ void launch_stage_g2p(const KArgs& a, hipStream_t s) {
 void launch_stage_advect(const KArgs& a, hipStream_t s) {
   if (a.n) k_stage_advect<<<nblk(a.n), TPB, 0, s>>>(a);
 }
-void launch_scatter_field(float* dst, int64_t stride, const float* src, const int* pid, int n, int comps,
+void launch_scatter_field(float* dst, int64_t stride, const float* src, const int* pid, int p0, int np, int comps,
                           hipStream_t s) {
-  if (n) k_scatter_field<<<nblk(n), TPB, 0, s>>>(dst, stride, src, pid, n, comps);
+  if (np) k_scatter_field<<<nblk(np), TPB, 0, s>>>(dst, stride, src, pid, p0, np, comps);
 }
-void launch_gather_field(float* dst, const float* src, int64_t stride, const int* pid, int n, int comps,
+void launch_gather_field(float* dst, const float* src, int64_t stride, const int* pid, int p0, int np, int comps,
                          hipStream_t s) {
-  if (n) k_gather_field<<<nblk(n), TPB, 0, s>>>(dst, src, stride, pid, n, comps);
+  if (np) k_gather_field<<<nblk(np), TPB, 0, s>>>(dst, src, stride, pid, p0, np, comps);
 }
 void launch_iota(int* p, int n, hipStream_t s) {
   if (n) k_iota<<<nblk(n), TPB, 0, s>>>(p, n);
@@ -335,8 +363,12 @@ void launch_iota(int* p, int n, hipStream_t s) {
 void launch_fill_int(int* p, int64_t n, int v, hipStream_t s) {
   if (n) k_fill_int<<<nblk(n), TPB, 0, s>>>(p, n, v);
 }
-void launch_grid_to_dense(const KArgs& a, float* m, float* vel, float* force, uint8_t* active, hipStream_t s) {
-  k_grid_to_dense<<<nblk(a.nnodes), TPB, 0, s>>>(a, m, vel, force, active);
+void launch_grid_to_dense(const KArgs& a, const Scene& S, float* m, float* vel, float* force, uint8_t* active,
+                         hipStream_t s) {
+  k_grid_to_dense<<<nblk(S.nblocks * 64), TPB, 0, s>>>(a, S, m, vel, force, active);
+}
+void launch_volumes(const KArgs& a, hipStream_t s) {
+  if (a.n) k_volumes<<<nblk(a.n), TPB, 0, s>>>(a);
 }
 
 }  // namespace mpm

@@ -133,6 +133,24 @@ MPM_HD float div_rcp(float a, float b, float rb) {
   return fmaf(r, rb, q);
 }
 
+// glm's inverse of d_in * identity (compute_inverse, cofactors times one
+// division) for d_in > 0: the diagonal is d = d_in^2 / d_in^3 and the
+// cofactor signs leave these signed zeros off the diagonal.  Built from the
+// one scalar, the matrix costs no registers.
+MPM_HD M3 dinv_matrix(float d) {
+  M3 r;
+  r.c[0][0] = d;
+  r.c[1][0] = -0.0f;
+  r.c[2][0] = 0.0f;
+  r.c[0][1] = -0.0f;
+  r.c[1][1] = d;
+  r.c[2][1] = -0.0f;
+  r.c[0][2] = 0.0f;
+  r.c[1][2] = -0.0f;
+  r.c[2][2] = d;
+  return r;
+}
+
 // -------------------------------------------------------- cubic B-spline
 // N(u) of material_point_method.hpp:20-31.  The reference's double literals
 // promote the polynomial to double before the final rounding; FP64 on gfx950

@@ -16,6 +16,7 @@ PKG_DIR = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libmpm_hip.so")
 
 MPM_OK = 0
+MPM_ERR_ARG, MPM_ERR_HIP, MPM_ERR_BOUNDS, MPM_ERR_NONFINITE, MPM_ERR_STATE, MPM_ERR_NOMEM = 1, 2, 3, 4, 5, 6
 ERRORS = {1: "MPM_ERR_ARG", 2: "MPM_ERR_HIP", 3: "MPM_ERR_BOUNDS", 4: "MPM_ERR_NONFINITE", 5: "MPM_ERR_STATE",
           6: "MPM_ERR_NOMEM"}
 COMPOSE_LITERAL, COMPOSE_TEXTBOOK = 0, 1
@@ -65,6 +66,9 @@ EXPORTS = [
     "mpm_set_particles", "mpm_set_state", "mpm_set_colliders", "mpm_init_volumes", "mpm_stage",
     "mpm_substeps", "mpm_sync", "mpm_get_particles", "mpm_get_grid", "mpm_profile_begin", "mpm_profile_end",
     "mpm_count_active", "mpm_debug_bin", "mpm_copy_probe", "mpm_use_graphs",
+    "mpm_create_batch", "mpm_num_scenes", "mpm_scene_set_particles", "mpm_scene_set_state",
+    "mpm_scene_set_colliders", "mpm_scene_get_particles", "mpm_scene_get_grid", "mpm_scene_status",
+    "mpm_set_tuning", "mpm_get_tuning", "mpm_autotune",
 ]
 KERNELS = {"p2g": 0, "grid": 1, "g2p": 2, "bin": 3, "fused": 4}
 NKERNELS = 8
@@ -95,6 +99,17 @@ def lib():
         L.mpm_get_grid.argtypes = [vp, _F, _F, _F, C.POINTER(C.c_uint8)]
         L.mpm_profile_begin.argtypes = [vp, C.c_int32]
         L.mpm_use_graphs.argtypes = [vp, C.c_int32]
+        L.mpm_create_batch.argtypes = [C.POINTER(Params), C.POINTER(C.c_int32), C.c_int32, C.POINTER(vp)]
+        L.mpm_num_scenes.argtypes = [vp]
+        L.mpm_scene_set_particles.argtypes = [vp, C.c_int32, _F, _F, _F, _F]
+        L.mpm_scene_set_state.argtypes = [vp, C.c_int32, _F, _F, _F, _F]
+        L.mpm_scene_set_colliders.argtypes = [vp, C.c_int32, C.POINTER(Collider), C.c_int32]
+        L.mpm_scene_get_particles.argtypes = [vp, C.c_int32] + [_F] * 6
+        L.mpm_scene_get_grid.argtypes = [vp, C.c_int32, _F, _F, _F, C.POINTER(C.c_uint8)]
+        L.mpm_scene_status.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.mpm_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
+        L.mpm_get_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.mpm_autotune.argtypes = [vp, C.c_float, C.c_int32, C.POINTER(C.c_double)]
         L.mpm_profile_end.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.mpm_count_active.argtypes = [vp, C.POINTER(C.c_int64)]
         L.mpm_debug_bin.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]
@@ -132,16 +147,27 @@ def collider(translation, rotation=(1.0, 0.0, 0.0, 0.0), scale=(1.0, 1.0, 1.0), 
 
 
 class Simulation:
-    """One device context (mpm_ctx)."""
+    """One device context (mpm_ctx): one scene, or a batch of scenes when
+    `params` and `n` are sequences (scene s: params[s], n[s] particles)."""
 
-    def __init__(self, params: Params, n: int):
+    def __init__(self, params, n):
         self.L = lib()
-        self.n = int(n)
-        self.params = params
-        self.dims = tuple(params.dims)
         h = C.c_void_p()
-        self._check(self.L.mpm_create(C.byref(params), self.n, C.byref(h)), h)
+        if isinstance(params, (list, tuple)):
+            self.scene_params = list(params)
+            self.scene_n = [int(k) for k in n]
+            arr = (Params * len(params))(*params)
+            cnt = (C.c_int32 * len(params))(*self.scene_n)
+            self._check(self.L.mpm_create_batch(arr, cnt, len(params), C.byref(h)), h)
+        else:
+            self.scene_params = [params]
+            self.scene_n = [int(n)]
+            self._check(self.L.mpm_create(C.byref(params), int(n), C.byref(h)), h)
         self.ctx = h
+        self.n = sum(self.scene_n)
+        self.params = self.scene_params[0]
+        self.dims = tuple(self.params.dims)
+        self.offsets = np.r_[0, np.cumsum(self.scene_n)].astype(np.int64)
 
     def _check(self, rc, h=None):
         if rc != MPM_OK:
@@ -224,6 +250,47 @@ class Simulation:
                    FE=np.zeros((n, 9), np.float32), FP=np.zeros((n, 9), np.float32), V0=np.zeros(n, np.float32))
         self._check(self.L.mpm_get_particles(self.ctx, *[_fp(out[k]) for k in ("x", "v", "B", "FE", "FP", "V0")]))
         return out
+
+    # ---- batched scenes
+    @property
+    def nscenes(self):
+        return len(self.scene_n)
+
+    def scene_set_particles(self, scene, x, v, m, E0=None):
+        x, v, m, E0 = _f32(x), _f32(v), _f32(m), _f32(E0)
+        self._check(self.L.mpm_scene_set_particles(self.ctx, int(scene), _fp(x), _fp(v), _fp(m), _fp(E0)))
+
+    def scene_set_colliders(self, scene, colliders):
+        arr = (Collider * max(1, len(colliders)))(*colliders)
+        self._check(self.L.mpm_scene_set_colliders(self.ctx, int(scene), arr, len(colliders)))
+
+    def scene_particles(self, scene):
+        n = self.scene_n[scene]
+        out = dict(x=np.zeros((n, 3), np.float32), v=np.zeros((n, 3), np.float32), B=np.zeros((n, 9), np.float32),
+                   FE=np.zeros((n, 9), np.float32), FP=np.zeros((n, 9), np.float32), V0=np.zeros(n, np.float32))
+        self._check(self.L.mpm_scene_get_particles(self.ctx, int(scene),
+                                                   *[_fp(out[k]) for k in ("x", "v", "B", "FE", "FP", "V0")]))
+        return out
+
+    def scene_status(self):
+        """Per-scene MPM_OK / MPM_ERR_* since the last status read."""
+        st = (C.c_int32 * self.nscenes)()
+        self._check(self.L.mpm_scene_status(self.ctx, st))
+        return list(st)
+
+    def set_tuning(self, resort_every, block_grid):
+        self._check(self.L.mpm_set_tuning(self.ctx, int(resort_every), int(block_grid)))
+
+    def tuning(self):
+        r, g = C.c_int32(), C.c_int32()
+        self._check(self.L.mpm_get_tuning(self.ctx, C.byref(r), C.byref(g)))
+        return r.value, g.value
+
+    def autotune(self, dt, steps=16):
+        """-> best ms per substep; keeps the fastest (resort_every, block_grid)."""
+        ms = C.c_double()
+        self._check(self.L.mpm_autotune(self.ctx, float(dt), int(steps), C.byref(ms)))
+        return ms.value
 
     def grid(self, force=False):
         cells = int(np.prod(self.dims))

@@ -135,3 +135,28 @@ def reference_colliders():
         dict(translation=(0.0, 0.3, 0.5), rotation=q45, scale=(0.2, 0.2, 0.3)),
         dict(translation=(1.0, 0.3, 0.5), rotation=q45, scale=(0.2, 0.2, 0.3)),
     ]
+
+
+def scene_batch(seed=3, nscenes=64, n_range=(4096, 262144), e_range=(5e4, 5e5), v_range=3.0, ppc=8.0,
+                small_grid=64, large_grid=128, split=32768, dt=1e-4, steps=500):
+    """Many independent snowballs stepped together: per scene N_s
+    log-uniform in n_range, grid small_grid^3 for N_s <= split else
+    large_grid^3, a sphere whose radius gives `ppc` particles per cell, centre
+    uniform in [0.3, 0.7]^3, E0 log-uniform in e_range, initial velocity
+    uniform in [-v_range, v_range]^3, uniform density RHO.  Returns a list of
+    single-scene dicts (same keys as snowball())."""
+    rng = _rng(seed)
+    out = []
+    for s in range(nscenes):
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        grid = small_grid if n <= split else large_grid
+        h = 1.0 / grid
+        radius = (3.0 * n * h ** 3 / (ppc * 4.0 * math.pi)) ** (1.0 / 3.0)
+        centre = rng.uniform(0.3, 0.7, 3)
+        E = float(math.exp(rng.uniform(math.log(e_range[0]), math.log(e_range[1]))))
+        v0 = rng.uniform(-v_range, v_range, 3).astype(np.float32)
+        x = _ball(rng, n, centre, radius)
+        mp = RHO * (4.0 / 3.0) * math.pi * radius ** 3 / n
+        out.append(dict(x=x, v=np.tile(v0, (n, 1)), m=np.full(n, mp, np.float32), E0=np.full(n, E, np.float32),
+                        dims=(grid, grid, grid), h=h, dt=dt, steps=steps, params=material(E=E)))
+    return out

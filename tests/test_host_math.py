@@ -123,3 +123,13 @@ def test_f_update_bit_exact_against_oracle(hc, compose):
             assert rc == 0 and orc == 0
             np.testing.assert_array_equal(FEo, FEr)
             np.testing.assert_array_equal(FPo, FPr)
+
+
+def test_dp_inverse_pattern_all_spacings(hc):
+    """The kernels rebuild D^-1 = inverse(mat3(1) * (1/3) * h * h) from its
+    diagonal with a fixed pattern of signed zeros off the diagonal; that must
+    be glm's result bit for bit."""
+    hc.hc_dinv_pattern.argtypes = [C.c_float]
+    hc.hc_dinv_pattern.restype = C.c_int
+    for h in (1 / 20, 0.05, 1 / 64, 1 / 96, 1 / 128, 1 / 256, 0.0123, 0.3, 2.0):
+        assert hc.hc_dinv_pattern(h) == 1, h
