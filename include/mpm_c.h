@@ -172,6 +172,19 @@ int mpm_set_tuning(mpm_ctx* ctx, int32_t resort_every, int32_t block_grid);
 int mpm_get_tuning(mpm_ctx* ctx, int32_t* resort_every, int32_t* block_grid);
 int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms_per_substep);
 
+/* ---- reverse mode (no reference counterpart: the reference has no
+ * gradients).  mpm_adjoint_forward runs nsteps substeps from the current
+ * state, keeping a checkpoint every `checkpoint_every` substeps (device
+ * memory: 132 bytes per particle per checkpoint).  mpm_adjoint_backward takes
+ * dL/dx and dL/dv of the final particles (n x 3, caller order; NULL = 0) and
+ * returns dL/dx0, dL/dv0 (n x 3) and dL/dE0 (n) of the state the forward
+ * started from (NULL skips), recomputing each segment from its checkpoint; it
+ * leaves the context at that initial state.  TEXTBOOK composition and domain
+ * walls only (MPM_ERR_STATE otherwise); V0 is held fixed, so dL/dx0 leaves
+ * out V0's dependence on the initial positions. */
+int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t checkpoint_every);
+int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* gx0, float* gv0, float* gE0);
+
 /* ---- measurement (no reference counterpart: the reference times stages
  * with millisecond std::chrono stamps, realtime-deformations/utils.h:138-154,
  * main.cpp:192-225) ----------------------------------------------------- */

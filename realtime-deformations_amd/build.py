@@ -28,6 +28,7 @@ SOURCES = [
     "mpm_fused.hip",
     "mpm_bin.hip",
     "mpm_capi.hip",
+    "mpm_adjoint.hip",
 ]
 FACADE = os.path.join(PKG, "facade")
 DEMO = os.path.join(LIBDIR, "facade_demo")

@@ -1,0 +1,450 @@
+// Reverse-mode kernels of the MPM snow substep (gfx950): the adjoint of one
+// substep, stage by stage, for the TEXTBOOK composition with domain walls.
+//
+// Forward substep n (material_point_method.cpp:94-350 in main.cpp's order,
+// with the force impulse folded into the momentum as in the fused pass):
+//   P2G   m_i = sum_p w m,  P_i = sum_p w (m v + Q (X_i - x)),
+//         Q = m B D^-1 - dt V0 D^-1 P(F_E, F_P, E) F_E^T
+//   grid  v_i = walls(P_i / m_i + dt g)
+//   F     F^ = (I + dt B D^-1) F_E,  F_tot = F^ F_P,
+//         F_E' = V clamp(S) U^T (F^ = V S U^T),  F_P' = F_E'^-1 F_tot
+//   G2P   v' = sum_i w v_i,  B' = sum_i w v_i (X_i - x)^T
+//   x'    = clamp(x + dt v')
+// The reverse runs these backwards from the stored state of substep n:
+//   k_adj_grid_fwd  grid velocities of substep n from the re-run P2G sums
+//   k_adj_g2p       advection and G2P: node velocity adjoints (atomics) and
+//                   the position adjoint through the weights
+//   k_adj_grid      wall mask and v = P/m + dt g: adjoints of (m_i, P_i)
+//   k_adj_p2g       P2G (gathered), stress, hardening and the F update:
+//                   adjoints of x, v, B, F_E, F_P of substep n and of E0
+// Derivatives of functions of singular values use the divided-difference
+// form in the singular basis (stable for repeated singular values); the
+// polar rotation is the case f = 1.  V0 is held fixed.
+#include "mpm_adjoint.hpp"
+
+namespace mpm {
+namespace {
+
+constexpr int TPB = 256;
+inline int nblk(int64_t n) { return (int)((n + TPB - 1) / TPB); }
+
+__device__ __forceinline__ float fld(const float* P, int64_t stride, int f, int p) { return P[(int64_t)f * stride + p]; }
+__device__ __forceinline__ M3 ld_m3(const float* P, int64_t stride, int f, int p) {
+  M3 m;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) m.c[c][r] = P[(int64_t)(f + 3 * c + r) * stride + p];
+  return m;
+}
+__device__ __forceinline__ void st_m3(float* P, int64_t stride, int f, int p, const M3& m) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) P[(int64_t)(f + 3 * c + r) * stride + p] = m.c[c][r];
+}
+
+// 1-D weights and their derivatives d/dx on the stencil base-1 .. base+2
+struct AxisD {
+  int base;
+  float w[4], dw[4];
+};
+__device__ __forceinline__ float dweight(float u) {  // dN/du
+  const float a = fabsf(u);
+  if (a < 1.0f) return u * (1.5f * a - 2.0f);
+  if (a < 2.0f) {
+    const float t = 2.0f - a;
+    return (u < 0.0f ? 0.5f : -0.5f) * t * t;
+  }
+  return 0.0f;
+}
+__device__ __forceinline__ AxisD axis_d(float x, const Scene& S, int hi) {
+  const Axis a = axis_weights(x, S.h, S.inv_h, 1, hi);
+  AxisD r;
+  r.base = a.base;
+  const float q = div_rcp(x, S.h, S.inv_h);
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    r.w[d] = a.w[d];
+    r.dw[d] = dweight(q - (float)(a.base - 1 + d)) * S.inv_h;
+  }
+  return r;
+}
+
+__device__ __forceinline__ M3 mzero() { return mdiag(0.0f); }
+__device__ __forceinline__ float mdot(const M3& a, const M3& b) {
+  float s = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) s = fmaf(a.c[c][r], b.c[c][r], s);
+  return s;
+}
+// row-major math view helpers: A(r, c) = A.c[c][r]
+__device__ __forceinline__ M3 from_rows(const float (&a)[3][3]) {
+  M3 m;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m.c[c][r] = a[r][c];
+  return m;
+}
+
+// Adjoint of F -> Lb diag(f(s)) Rb^T, a function of the singular values of
+// F = Lb diag(s) Rb^T (f' = fp): G = Lb^T gOut Rb, then in the singular basis
+// H_ii = f'_i G_ii, H_ij = a (G_ij + G_ji)/2 + b (G_ij - G_ji)/2 with the
+// divided differences a = (f_i - f_j)/(s_i - s_j), b = (f_i + f_j)/(s_i + s_j);
+// returns Lb H Rb^T.
+__device__ __forceinline__ M3 spectral_adjoint(const M3& Lb, const M3& Rb, const float s[3], const float f[3],
+                                               const float fp[3], const M3& gOut) {
+  const M3 G = mmul(mmul(mtrans(Lb), gOut), Rb);  // G(r,c) = G.c[c][r]
+  M3 H;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (i == j) {
+        H.c[i][i] = fp[i] * G.c[i][i];
+      } else {
+        const float gij = G.c[j][i], gji = G.c[i][j];
+        const float ds = s[i] - s[j];
+        const float a = fabsf(ds) > 1e-6f * fmaxf(1.0f, fmaxf(s[i], s[j])) ? (f[i] - f[j]) / ds : 0.5f * (fp[i] + fp[j]);
+        const float ss = s[i] + s[j];
+        const float b = ss > 0.0f ? (f[i] + f[j]) / ss : 0.0f;
+        H.c[j][i] = 0.5f * a * (gij + gji) + 0.5f * b * (gij - gji);
+      }
+    }
+  return mmul(mmul(Lb, H), mtrans(Rb));
+}
+
+// SVD of F as F = Lb diag(s) Rb^T from the Eigen-convention solver of F^T
+// (F^T = U S V^T  =>  F = V S U^T: Lb = V, Rb = U).
+__device__ __forceinline__ bool svd_of(const M3& F, M3& Lb, float (&s)[3], M3& Rb) {
+  float M[3][3], U[3][3], V[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) M[i][j] = F.c[i][j];  // M(i,j) = F(j,i)
+  if (!jacobi_svd3<float>(M, U, s, V)) return false;
+  Lb = from_rows(V);
+  Rb = from_rows(U);
+  return true;
+}
+
+// ------------------------------------------------------------------ kernels
+
+// grid velocities of the re-run substep from its P2G sums (gacc kept)
+__global__ void k_adj_grid_fwd(KArgs a) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nnodes) return;
+  const float4 acc = a.gacc[g];
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (acc.x != 0.0f) {
+    const Scene S = a.scenes[a.blk_scene[g >> 6]];
+    F3 v = f3(acc.y / acc.x + a.dt * S.g[0], acc.z / acc.x + a.dt * S.g[1], acc.w / acc.x + a.dt * S.g[2]);
+    if (S.bc_mode & MPM_BC_DOMAIN) {
+      const int64_t lb = (g >> 6) - S.blk0;
+      const int l = (int)(g & 63);
+      const int bk = (int)(lb % S.nb[2]);
+      const int64_t t2 = lb / S.nb[2];
+      const int bj = (int)(t2 % S.nb[1]), bi = (int)(t2 / S.nb[1]);
+      v = domain_walls(bi * 4 + (l >> 4), bj * 4 + ((l >> 2) & 3), bk * 4 + (l & 3), S.dims[0], S.dims[1], S.dims[2], v);
+    }
+    out = make_float4(v.x, v.y, v.z, acc.x);
+  }
+  a.gvel[g] = out;
+}
+
+// advection and G2P adjoint.  St: stored state of substep n (slot order);
+// Gin: adjoints of substep n+1 (x, v, B at AX, AV, AB); Gout.x: position
+// adjoint (partial); gnode: node velocity adjoints (atomics).
+__global__ void k_adj_g2p(KArgs a, const float* St, const float* Gin, float* Gout, float4* gnode) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const Scene S = a.scenes[scene_of_slot(a.scenes, a.nscenes, p)];
+  const int64_t sd = a.stride;
+  const F3 x = f3(fld(St, sd, FX, p), fld(St, sd, FX + 1, p), fld(St, sd, FX + 2, p));
+  AxisD ax[3];
+  ax[0] = axis_d(x.x, S, S.dims[0] - 3);
+  ax[1] = axis_d(x.y, S, S.dims[1] - 3);
+  ax[2] = axis_d(x.z, S, S.dims[2] - 3);
+  // forward G2P velocity for the advection clamp
+  F3 vn = f3(0.f, 0.f, 0.f);
+  for (int ia = 0; ia < 4; ++ia)
+    for (int ib = 0; ib < 4; ++ib)
+      for (int ic = 0; ic < 4; ++ic) {
+        const float w = ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic];
+        const float4 gv = a.gvel[node_index(ax[0].base - 1 + ia, ax[1].base - 1 + ib, ax[2].base - 1 + ic, S)];
+        vn = vn + f3(gv.x, gv.y, gv.z) * w;
+      }
+  const float xs[3] = {x.x, x.y, x.z}, vs[3] = {vn.x, vn.y, vn.z};
+  float gx[3], gv[3];
+  for (int k = 0; k < 3; ++k) {
+    const float xn = xs[k] + vs[k] * a.dt;
+    const float lo = 3.0f * S.h, hi = (float)(S.dims[k] - 3) * S.h;
+    const float mask = (xn > lo && xn < hi) ? 1.0f : 0.0f;
+    const float gxn = fld(Gin, sd, AX + k, p);
+    gx[k] = gxn * mask;
+    gv[k] = fld(Gin, sd, AV + k, p) + a.dt * gxn * mask;
+  }
+  const M3 gB = ld_m3(Gin, sd, AB, p);  // gB.c[c][r] = dL/dB(r, c)
+  for (int ia = 0; ia < 4; ++ia)
+    for (int ib = 0; ib < 4; ++ib)
+      for (int ic = 0; ic < 4; ++ic) {
+        const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib, k = ax[2].base - 1 + ic;
+        const float w = ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic];
+        const F3 dw = f3(ax[0].dw[ia] * ax[1].w[ib] * ax[2].w[ic], ax[0].w[ia] * ax[1].dw[ib] * ax[2].w[ic],
+                         ax[0].w[ia] * ax[1].w[ib] * ax[2].dw[ic]);
+        const int64_t g = node_index(i, j, k, S);
+        const float4 gv4 = a.gvel[g];
+        const F3 vi = f3(gv4.x, gv4.y, gv4.z);
+        const F3 d = f3((float)i * S.h - x.x, (float)j * S.h - x.y, (float)k * S.h - x.z);
+        // u = gv' + gB' d (what node velocity v_i multiplies, times w)
+        const F3 gBd = f3(gB.c[0][0] * d.x + gB.c[1][0] * d.y + gB.c[2][0] * d.z,
+                          gB.c[0][1] * d.x + gB.c[1][1] * d.y + gB.c[2][1] * d.z,
+                          gB.c[0][2] * d.x + gB.c[1][2] * d.y + gB.c[2][2] * d.z);
+        const F3 u = f3(gv[0], gv[1], gv[2]) + gBd;
+        const float su = dot(vi, u);
+        // gB'^T v_i
+        const F3 gBtv = f3(gB.c[0][0] * vi.x + gB.c[0][1] * vi.y + gB.c[0][2] * vi.z,
+                           gB.c[1][0] * vi.x + gB.c[1][1] * vi.y + gB.c[1][2] * vi.z,
+                           gB.c[2][0] * vi.x + gB.c[2][1] * vi.y + gB.c[2][2] * vi.z);
+        gx[0] += dw.x * su - w * gBtv.x;
+        gx[1] += dw.y * su - w * gBtv.y;
+        gx[2] += dw.z * su - w * gBtv.z;
+        float* gn = reinterpret_cast<float*>(gnode + g);
+        atomicAdd(gn + 1, w * u.x);
+        atomicAdd(gn + 2, w * u.y);
+        atomicAdd(gn + 3, w * u.z);
+      }
+  for (int k = 0; k < 3; ++k) Gout[(int64_t)(AX + k) * sd + p] = gx[k];
+}
+
+// wall mask and v = P/m + dt g: node adjoints (dL/dm, dL/dP) in place
+__global__ void k_adj_grid(KArgs a, float4* gnode) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nnodes) return;
+  const float4 acc = a.gacc[g];
+  float4 gn = gnode[g];
+  if (acc.x == 0.0f) {
+    gnode[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const Scene S = a.scenes[a.blk_scene[g >> 6]];
+  const F3 vpre = f3(acc.y / acc.x + a.dt * S.g[0], acc.z / acc.x + a.dt * S.g[1], acc.w / acc.x + a.dt * S.g[2]);
+  float mk[3] = {1.f, 1.f, 1.f};
+  if (S.bc_mode & MPM_BC_DOMAIN) {
+    const int64_t lb = (g >> 6) - S.blk0;
+    const int l = (int)(g & 63);
+    const int bk = (int)(lb % S.nb[2]);
+    const int64_t t2 = lb / S.nb[2];
+    const int bj = (int)(t2 % S.nb[1]), bi = (int)(t2 / S.nb[1]);
+    const int i = bi * 4 + (l >> 4), j = bj * 4 + ((l >> 2) & 3), k = bk * 4 + (l & 3);
+    // domain_walls: floor band sticky, other bands clamp the outward component
+    if (j < 3) {
+      mk[0] = mk[1] = mk[2] = 0.f;
+    } else {
+      if ((i < 3 && vpre.x < 0.f) || (i >= S.dims[0] - 3 && vpre.x > 0.f)) mk[0] = 0.f;
+      if (j >= S.dims[1] - 3 && vpre.y > 0.f) mk[1] = 0.f;
+      if ((k < 3 && vpre.z < 0.f) || (k >= S.dims[2] - 3 && vpre.z > 0.f)) mk[2] = 0.f;
+    }
+  }
+  const F3 gvp = f3(gn.y * mk[0], gn.z * mk[1], gn.w * mk[2]);
+  const float inv_m = 1.0f / acc.x;
+  const F3 gP = gvp * inv_m;
+  const float gm = -(acc.y * gvp.x + acc.z * gvp.y + acc.w * gvp.z) * inv_m * inv_m;
+  gnode[g] = make_float4(gm, gP.x, gP.y, gP.z);
+}
+
+// P2G, stress, hardening and F-update adjoint for substep n.  St: state of
+// substep n; Gin: adjoints of substep n+1 (F_E, F_P); Gout: on entry the
+// partial position adjoint, on exit all adjoints of substep n; gE += dL/dE0.
+__global__ void k_adj_p2g(KArgs a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const Scene S = a.scenes[scene_of_slot(a.scenes, a.nscenes, p)];
+  const int64_t sd = a.stride;
+  const F3 x = f3(fld(St, sd, FX, p), fld(St, sd, FX + 1, p), fld(St, sd, FX + 2, p));
+  const F3 v = f3(fld(St, sd, FV, p), fld(St, sd, FV + 1, p), fld(St, sd, FV + 2, p));
+  const M3 B = ld_m3(St, sd, FB, p), FE = ld_m3(St, sd, FFE, p), FP = ld_m3(St, sd, FFP, p);
+  const float m = fld(a.P, sd, FM, p), V0 = fld(a.P, sd, FVOL, p), E = fld(a.P, sd, FE0, p);
+  const float dinv = S.dinv;
+  const M3 Dinv = dinv_matrix(dinv);
+  // forward stress pieces
+  const float nu = S.lame.nu, xi = S.lame.xi;
+  const float mu0 = E / S.lame.dmu, lam0 = (E * nu) / S.lame.dlam;
+  const float JP = mdet(FP);
+  const float hard = expf(xi * (1.0f - JP));
+  const float mu = mu0 * hard, lam = lam0 * hard;
+  M3 Lb, Rb;
+  float sv[3];
+  svd_of(FE, Lb, sv, Rb);
+  const M3 R = mmul(Lb, mtrans(Rb));
+  const float J = mdet(FE);
+  const M3 K = mtrans(minverse(FE));  // F_E^-T
+  const float cJ = (J - 1.0f) * J;
+  const M3 Pk = madd(mscale(msub(FE, R), 2.0f * mu), mscale(K, lam * cJ));
+  const M3 A = mmul(mscale(Dinv, V0), mmul(Pk, mtrans(FE)));
+  M3 Q;
+  const M3 BD = mmul(B, Dinv);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) Q.c[c][r] = m * BD.c[c][r] - a.dt * A.c[c][r];
+  // gather the node adjoints
+  AxisD ax[3];
+  ax[0] = axis_d(x.x, S, S.dims[0] - 3);
+  ax[1] = axis_d(x.y, S, S.dims[1] - 3);
+  ax[2] = axis_d(x.z, S, S.dims[2] - 3);
+  F3 gv = f3(0.f, 0.f, 0.f);
+  F3 gx = f3(fld(Gout, sd, AX, p), fld(Gout, sd, AX + 1, p), fld(Gout, sd, AX + 2, p));
+  M3 gQ = mzero();
+  const F3 mv = v * m;
+  for (int ia = 0; ia < 4; ++ia)
+    for (int ib = 0; ib < 4; ++ib)
+      for (int ic = 0; ic < 4; ++ic) {
+        const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib, k = ax[2].base - 1 + ic;
+        const float w = ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic];
+        const F3 dw = f3(ax[0].dw[ia] * ax[1].w[ib] * ax[2].w[ic], ax[0].w[ia] * ax[1].dw[ib] * ax[2].w[ic],
+                         ax[0].w[ia] * ax[1].w[ib] * ax[2].dw[ic]);
+        const float4 gn = gnode[node_index(i, j, k, S)];
+        const F3 gP = f3(gn.y, gn.z, gn.w);
+        const F3 d = f3((float)i * S.h - x.x, (float)j * S.h - x.y, (float)k * S.h - x.z);
+        const F3 Qd = mvec(Q, d);
+        const float s = m * gn.x + dot(gP, mv + Qd);
+        // Q^T gP
+        const F3 QtgP = f3(Q.c[0][0] * gP.x + Q.c[0][1] * gP.y + Q.c[0][2] * gP.z,
+                           Q.c[1][0] * gP.x + Q.c[1][1] * gP.y + Q.c[1][2] * gP.z,
+                           Q.c[2][0] * gP.x + Q.c[2][1] * gP.y + Q.c[2][2] * gP.z);
+        gx = gx + dw * s - QtgP * w;
+        gv = gv + gP * (w * m);
+        // gQ(r, c) += w gP_r d_c
+        const float dd[3] = {d.x, d.y, d.z}, gp[3] = {gP.x, gP.y, gP.z};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) gQ.c[c][r] += w * gp[r] * dd[c];
+      }
+  // Q = m B D - dt A
+  M3 gB = mscale(mmul(gQ, mtrans(Dinv)), m);
+  const M3 gA = mscale(gQ, -a.dt);
+  // A = V0 D P F_E^T
+  const M3 gPk = mscale(mmul(mmul(mtrans(Dinv), gA), FE), V0);
+  M3 gFE = mmul(mtrans(gA), mscale(mmul(Dinv, Pk), V0));
+  // P = 2 mu (F - R) + lam c(J) K
+  const float gmu = 2.0f * mdot(gPk, msub(FE, R));
+  const float glam = cJ * mdot(gPk, K);
+  gFE = madd(gFE, mscale(gPk, 2.0f * mu));
+  {
+    const float one[3] = {1.f, 1.f, 1.f}, zero[3] = {0.f, 0.f, 0.f};
+    gFE = madd(gFE, spectral_adjoint(Lb, Rb, sv, one, zero, mscale(gPk, -2.0f * mu)));
+  }
+  const M3 gK = mscale(gPk, lam * cJ);
+  const float gJ = lam * mdot(gPk, K) * (2.0f * J - 1.0f);
+  gFE = madd(gFE, mscale(K, gJ * J));
+  gFE = msub(gFE, mmul(mmul(K, mtrans(gK)), K));
+  // mu = E / dmu * hard, lam = E nu / dlam * hard, hard = exp(xi (1 - det F_P))
+  gE[p] += hard * (gmu / S.lame.dmu + glam * nu / S.lame.dlam);
+  const float ghard = gmu * mu0 + glam * lam0;
+  M3 gFP = mscale(mtrans(minverse(FP)), -xi * hard * ghard * JP);
+  // F update (TEXTBOOK)
+  {
+    const M3 gFEn = ld_m3(Gin, sd, AFE, p), gFPn = ld_m3(Gin, sd, AFP, p);
+    const M3 Fh = mmul(madd(mdiag(1.0f), mscale(BD, a.dt)), FE);
+    const M3 Ftot = mmul(Fh, FP);
+    M3 Lh, Rh;
+    float sh[3];
+    svd_of(Fh, Lh, sh, Rh);
+    float f[3], fp[3];
+    for (int i = 0; i < 3; ++i) {
+      f[i] = fminf(fmaxf(sh[i], S.lo), S.hi);
+      fp[i] = (sh[i] > S.lo && sh[i] < S.hi) ? 1.0f : 0.0f;
+    }
+    M3 Sc = mdiag(0.0f);
+    for (int i = 0; i < 3; ++i) Sc.c[i][i] = f[i];
+    const M3 FEn = mmul(mmul(Lh, Sc), mtrans(Rh));
+    const M3 Y = minverse(FEn);
+    // F_P' = Y F_tot
+    const M3 gY = mmul(gFPn, mtrans(Ftot));
+    const M3 gFtot = mmul(mtrans(Y), gFPn);
+    const M3 gFEtot = msub(gFEn, mmul(mmul(mtrans(Y), gY), mtrans(Y)));
+    M3 gFh = spectral_adjoint(Lh, Rh, sh, f, fp, gFEtot);
+    gFh = madd(gFh, mmul(gFtot, mtrans(FP)));
+    gFP = madd(gFP, mmul(mtrans(Fh), gFtot));
+    // F^ = (I + dt B D) F_E
+    gFE = madd(gFE, mmul(mtrans(madd(mdiag(1.0f), mscale(BD, a.dt))), gFh));
+    gB = madd(gB, mscale(mmul(mmul(gFh, mtrans(FE)), mtrans(Dinv)), a.dt));
+  }
+  Gout[(int64_t)AX * sd + p] = gx.x;
+  Gout[(int64_t)(AX + 1) * sd + p] = gx.y;
+  Gout[(int64_t)(AX + 2) * sd + p] = gx.z;
+  Gout[(int64_t)AV * sd + p] = gv.x;
+  Gout[(int64_t)(AV + 1) * sd + p] = gv.y;
+  Gout[(int64_t)(AV + 2) * sd + p] = gv.z;
+  st_m3(Gout, sd, AB, p, gB);
+  st_m3(Gout, sd, AFE, p, gFE);
+  st_m3(Gout, sd, AFP, p, gFP);
+}
+
+// slot <-> caller order of `rows` consecutive rows
+__global__ void k_rows_to_slots(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int q = pid[p];
+  for (int f = 0; f < rows; ++f) dst[(int64_t)f * stride + p] = src[(int64_t)f * stride + q];
+}
+__global__ void k_rows_to_caller(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int q = pid[p];
+  for (int f = 0; f < rows; ++f) dst[(int64_t)f * stride + q] = src[(int64_t)f * stride + p];
+}
+__global__ void k_add_to_caller(float* dst, const float* src, const int* pid, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  dst[pid[p]] += src[p];
+}
+// n x comps interleaved <-> comps rows of `stride`
+__global__ void k_aos_to_rows(float* dst, const float* src, int n, int comps, int64_t stride) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int c = 0; c < comps; ++c) dst[(int64_t)c * stride + p] = src[(int64_t)p * comps + c];
+}
+__global__ void k_rows_to_aos(float* dst, const float* src, int n, int comps, int64_t stride) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int c = 0; c < comps; ++c) dst[(int64_t)p * comps + c] = src[(int64_t)c * stride + p];
+}
+
+}  // namespace
+
+void launch_adj_grid_fwd(const KArgs& a, hipStream_t s) { k_adj_grid_fwd<<<nblk(a.nnodes), TPB, 0, s>>>(a); }
+void launch_adj_g2p(const KArgs& a, const float* St, const float* Gin, float* Gout, float4* gnode, hipStream_t s) {
+  if (a.n) k_adj_g2p<<<nblk(a.n), TPB, 0, s>>>(a, St, Gin, Gout, gnode);
+}
+void launch_adj_grid(const KArgs& a, float4* gnode, hipStream_t s) {
+  k_adj_grid<<<nblk(a.nnodes), TPB, 0, s>>>(a, gnode);
+}
+void launch_adj_p2g(const KArgs& a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE,
+                    hipStream_t s) {
+  if (a.n) k_adj_p2g<<<nblk(a.n), TPB, 0, s>>>(a, St, Gin, Gout, gnode, gE);
+}
+void launch_rows_to_slots(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows,
+                          hipStream_t s) {
+  if (n) k_rows_to_slots<<<nblk(n), TPB, 0, s>>>(dst, src, pid, stride, n, rows);
+}
+void launch_rows_to_caller(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows,
+                           hipStream_t s) {
+  if (n) k_rows_to_caller<<<nblk(n), TPB, 0, s>>>(dst, src, pid, stride, n, rows);
+}
+void launch_add_to_caller(float* dst, const float* src, const int* pid, int n, hipStream_t s) {
+  if (n) k_add_to_caller<<<nblk(n), TPB, 0, s>>>(dst, src, pid, n);
+}
+void launch_aos_to_rows(float* dst, const float* src, int n, int comps, int64_t stride, hipStream_t s) {
+  if (n) k_aos_to_rows<<<nblk(n), TPB, 0, s>>>(dst, src, n, comps, stride);
+}
+void launch_rows_to_aos(float* dst, const float* src, int n, int comps, int64_t stride, hipStream_t s) {
+  if (n) k_rows_to_aos<<<nblk(n), TPB, 0, s>>>(dst, src, n, comps, stride);
+}
+
+}  // namespace mpm

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <new>
 
+#include "mpm_adjoint.hpp"
 #include "mpm_bin.hpp"
 #include "mpm_fused.hpp"
 #include "mpm_internal.hpp"
@@ -387,6 +388,8 @@ int mpm_destroy(mpm_ctx* ctx) {
                   ctx->gforce,  ctx->status,  ctx->dcount,  ctx->d_scenes, ctx->blk_scene};
   for (void* b : bufs) (void)hipFree(b);
   for (DevCollider* c : ctx->scoll) (void)hipFree(c);
+  for (float* c : ctx->adj_ckpt) (void)hipFree(c);
+  (void)hipFree(ctx->adj_const);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->ev_region)
     if (e) (void)hipEventDestroy(e);
@@ -954,6 +957,177 @@ int mpm_scene_get_grid(mpm_ctx* ctx, int32_t scene, float* m, float* vel, float*
 
 int mpm_get_grid(mpm_ctx* ctx, float* m, float* vel, float* force, uint8_t* active) {
   return mpm_scene_get_grid(ctx, 0, m, vel, force, active);
+}
+
+}  // extern "C"
+
+
+// ---------------------------------------------------------------- reverse mode
+namespace {
+
+int adjoint_supported(mpm_ctx* ctx) {
+  if (!ctx->particles_set) {
+    ctx->err = "no particles set";
+    return MPM_ERR_STATE;
+  }
+  if (ctx->prm.compose != MPM_COMPOSE_TEXTBOOK) {
+    ctx->err = "reverse mode needs the TEXTBOOK composition (LITERAL's U^T S V has no derivative at repeated "
+               "singular values)";
+    return MPM_ERR_STATE;
+  }
+  for (const Scene& S : ctx->scenes)
+    if ((S.bc_mode & MPM_BC_COLLIDERS) && S.ncoll > 0) {
+      ctx->err = "reverse mode supports the domain walls, not box colliders";
+      return MPM_ERR_STATE;
+    }
+  return MPM_OK;
+}
+
+// P rows [0, 33) and the constant rows back to the caller-ordered copies
+// (caller index = slot), ready to be binned.
+int restore_rows(mpm_ctx* ctx, const float* rows33) {
+  const int64_t sd = ctx->stride;
+  launch_iota(ctx->pid, ctx->n, ctx->stream);
+  MPM_HIP(hipMemcpyAsync(ctx->P, rows33, sizeof(float) * NA * sd, hipMemcpyDeviceToDevice, ctx->stream));
+  MPM_HIP(hipMemcpyAsync(ctx->P + (int64_t)FM * sd, ctx->adj_const, sizeof(float) * 3 * sd, hipMemcpyDeviceToDevice,
+                         ctx->stream));
+  ctx->since_sort = -1;
+  return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
+  if (check_ctx(ctx) || nsteps < 1 || every < 1) return MPM_ERR_ARG;
+  if (int rc = adjoint_supported(ctx)) return rc;
+  const int64_t sd = ctx->stride;
+  for (float* c : ctx->adj_ckpt) (void)hipFree(c);
+  ctx->adj_ckpt.clear();
+  ctx->adj_steps = 0;
+  if (!ctx->adj_const) MPM_HIP(hipMalloc(&ctx->adj_const, sizeof(float) * 3 * sd));
+  launch_rows_to_caller(ctx->adj_const, ctx->P + (int64_t)FM * sd, ctx->pid, sd, ctx->n, 3, ctx->stream);
+  const int nck = (nsteps + every - 1) / every;
+  for (int c = 0; c < nck; ++c) {
+    float* buf = nullptr;
+    MPM_HIP(hipMalloc(&buf, sizeof(float) * NA * sd));
+    ctx->adj_ckpt.push_back(buf);
+    launch_rows_to_caller(buf, ctx->P, ctx->pid, sd, ctx->n, NA, ctx->stream);
+    if (int rc = mpm_substeps(ctx, dt, std::min(every, nsteps - c * every))) return rc;
+  }
+  ctx->adj_dt = dt;
+  ctx->adj_steps = nsteps;
+  ctx->adj_every = every;
+  MPM_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* gx0, float* gv0, float* gE0) {
+  if (check_ctx(ctx)) return MPM_ERR_ARG;
+  if (ctx->adj_steps < 1) {
+    ctx->err = "mpm_adjoint_backward without mpm_adjoint_forward";
+    return MPM_ERR_STATE;
+  }
+  const int n = ctx->n;
+  const int64_t sd = ctx->stride;
+  const float dt = ctx->adj_dt;
+  const int every = ctx->adj_every, steps = ctx->adj_steps;
+  hipStream_t s = ctx->stream;
+  float *Gc = nullptr, *Gs = nullptr, *Gs2 = nullptr, *gEs = nullptr, *gEc = nullptr, *seg = nullptr, *tmp = nullptr;
+  float4* gnode = nullptr;
+  const size_t rows = sizeof(float) * NA * sd;
+  bool ok = true;
+  auto alloc = [&](auto** ptr, size_t bytes) {
+    if (ok && hipMalloc(reinterpret_cast<void**>(ptr), bytes) != hipSuccess) ok = false;
+  };
+  alloc(&Gc, rows);
+  alloc(&Gs, rows);
+  alloc(&Gs2, rows);
+  alloc(&gEs, sizeof(float) * sd);
+  alloc(&gEc, sizeof(float) * sd);
+  alloc(&seg, rows * (size_t)every);
+  alloc(&tmp, sizeof(float) * 3 * (size_t)std::max(1, n));
+  alloc(&gnode, sizeof(float4) * (size_t)ctx->nnodes);
+  int rc = MPM_OK;
+  auto cleanup = [&] {
+    void* b[] = {Gc, Gs, Gs2, gEs, gEc, seg, tmp, gnode};
+    for (void* p : b) (void)hipFree(p);
+  };
+  if (!ok) {
+    cleanup();
+    ctx->err = "device allocation failed";
+    return MPM_ERR_NOMEM;
+  }
+  MPM_HIP(hipMemsetAsync(Gc, 0, rows, s));
+  MPM_HIP(hipMemsetAsync(gEc, 0, sizeof(float) * sd, s));
+  if (gx) {
+    MPM_HIP(hipMemcpyAsync(tmp, gx, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+    launch_aos_to_rows(Gc + (int64_t)AX * sd, tmp, n, 3, sd, s);
+  }
+  if (gv) {
+    MPM_HIP(hipStreamSynchronize(s));
+    MPM_HIP(hipMemcpyAsync(tmp, gv, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+    launch_aos_to_rows(Gc + (int64_t)AV * sd, tmp, n, 3, sd, s);
+  }
+  const int keep_resort = ctx->resort_every;
+  ctx->resort_every = 1 << 30;  // one binning per segment: slots stay fixed inside it
+  const int nck = (steps + every - 1) / every;
+  for (int c = nck - 1; c >= 0 && !rc; --c) {
+    const int ns = std::min(every, steps - c * every);
+    if ((rc = restore_rows(ctx, ctx->adj_ckpt[c])) || (rc = rebin(ctx))) break;
+    // states of the segment's substeps, in its slot order
+    for (int k = 0; k < ns && !rc; ++k) {
+      MPM_HIP(hipMemcpyAsync(seg + (size_t)k * NA * sd, ctx->P, rows, hipMemcpyDeviceToDevice, s));
+      rc = mpm_substeps(ctx, dt, 1);
+    }
+    if (rc) break;
+    launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
+    MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
+    for (int k = ns - 1; k >= 0; --k) {
+      const float* St = seg + (size_t)k * NA * sd;
+      // re-run the P2G of this substep on a scratch copy (the pass also
+      // writes the F update back into the buffer it reads)
+      MPM_HIP(hipMemcpyAsync(ctx->P2, St, rows, hipMemcpyDeviceToDevice, s));
+      MPM_HIP(hipMemcpyAsync(ctx->P2 + (int64_t)FM * sd, ctx->P + (int64_t)FM * sd, sizeof(float) * 3 * sd,
+                             hipMemcpyDeviceToDevice, s));
+      launch_clear(ctx->gacc, ctx->nnodes, s);
+      BlockArgs ba = block_args(ctx, dt);
+      ba.a.P = ctx->P2;
+      launch_block_pass(ba, PASS_P2G, s);
+      KArgs a = make_args(ctx, dt);
+      launch_adj_grid_fwd(a, s);
+      launch_clear(gnode, ctx->nnodes, s);
+      launch_adj_g2p(a, St, Gs, Gs2, gnode, s);
+      launch_adj_grid(a, gnode, s);
+      launch_adj_p2g(a, St, Gs, Gs2, gnode, gEs, s);
+      std::swap(Gs, Gs2);
+    }
+    launch_rows_to_caller(Gc, Gs, ctx->pid, sd, n, NA, s);
+    launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
+    ctx->gacc_clean = false;
+  }
+  ctx->resort_every = keep_resort;
+  if (!rc) rc = restore_rows(ctx, ctx->adj_ckpt[0]);  // leave the initial state
+  ctx->gacc_clean = false;
+  if (!rc) {
+    MPM_HIP(hipGetLastError());
+    if (gx0) {
+      launch_rows_to_aos(tmp, Gc + (int64_t)AX * sd, n, 3, sd, s);
+      MPM_HIP(hipMemcpyAsync(gx0, tmp, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+      MPM_HIP(hipStreamSynchronize(s));
+    }
+    if (gv0) {
+      launch_rows_to_aos(tmp, Gc + (int64_t)AV * sd, n, 3, sd, s);
+      MPM_HIP(hipMemcpyAsync(gv0, tmp, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+      MPM_HIP(hipStreamSynchronize(s));
+    }
+    if (gE0) MPM_HIP(hipMemcpyAsync(gE0, gEc, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    MPM_HIP(hipStreamSynchronize(s));
+    rc = read_status(ctx);
+  }
+  cleanup();
+  return rc;
 }
 
 }  // extern "C"

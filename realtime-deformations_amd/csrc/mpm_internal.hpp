@@ -139,6 +139,13 @@ struct mpm_ctx {
   size_t events_used = 0;
   std::vector<int> event_kernel;    // kernel id per (start, stop) pair
   int64_t* dcount = nullptr;        // device counter scratch
+  // reverse mode: checkpoints of (x, v, B, F_E, F_P) in caller order every
+  // adj_every substeps of the last mpm_adjoint_forward, and the constant
+  // fields (m, V0, E0) in caller order
+  float adj_dt = 0.f;
+  int adj_steps = 0, adj_every = 0;
+  std::vector<float*> adj_ckpt;
+  float* adj_const = nullptr;
 };
 
 namespace mpm {

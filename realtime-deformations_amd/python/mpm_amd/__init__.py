@@ -68,7 +68,7 @@ EXPORTS = [
     "mpm_count_active", "mpm_debug_bin", "mpm_copy_probe", "mpm_use_graphs",
     "mpm_create_batch", "mpm_num_scenes", "mpm_scene_set_particles", "mpm_scene_set_state",
     "mpm_scene_set_colliders", "mpm_scene_get_particles", "mpm_scene_get_grid", "mpm_scene_status",
-    "mpm_set_tuning", "mpm_get_tuning", "mpm_autotune",
+    "mpm_set_tuning", "mpm_get_tuning", "mpm_autotune", "mpm_adjoint_forward", "mpm_adjoint_backward",
 ]
 KERNELS = {"p2g": 0, "grid": 1, "g2p": 2, "bin": 3, "fused": 4}
 NKERNELS = 8
@@ -110,6 +110,8 @@ def lib():
         L.mpm_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
         L.mpm_get_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.mpm_autotune.argtypes = [vp, C.c_float, C.c_int32, C.POINTER(C.c_double)]
+        L.mpm_adjoint_forward.argtypes = [vp, C.c_float, C.c_int32, C.c_int32]
+        L.mpm_adjoint_backward.argtypes = [vp, _F, _F, _F, _F, _F]
         L.mpm_profile_end.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.mpm_count_active.argtypes = [vp, C.POINTER(C.c_int64)]
         L.mpm_debug_bin.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]
@@ -291,6 +293,20 @@ class Simulation:
         ms = C.c_double()
         self._check(self.L.mpm_autotune(self.ctx, float(dt), int(steps), C.byref(ms)))
         return ms.value
+
+    # ---- reverse mode
+    def adjoint_forward(self, dt, nsteps, checkpoint_every=20):
+        """nsteps substeps from the current state, keeping checkpoints."""
+        self._check(self.L.mpm_adjoint_forward(self.ctx, float(dt), int(nsteps), int(checkpoint_every)))
+
+    def adjoint_backward(self, gx=None, gv=None):
+        """dL/dx, dL/dv of the final particles -> (dL/dx0, dL/dv0, dL/dE0);
+        leaves the context at the initial state."""
+        n = self.n
+        gx, gv = _f32(gx), _f32(gv)
+        gx0, gv0, gE0 = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
+        self._check(self.L.mpm_adjoint_backward(self.ctx, _fp(gx), _fp(gv), _fp(gx0), _fp(gv0), _fp(gE0)))
+        return gx0, gv0, gE0
 
     def grid(self, force=False):
         cells = int(np.prod(self.dims))

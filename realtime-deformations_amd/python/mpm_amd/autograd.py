@@ -1,0 +1,38 @@
+"""torch.autograd over the C ABI's reverse mode.
+
+    Rollout.apply(sim, dt, nsteps, every, x0, v0, E0) -> (x_T, v_T)
+
+runs `nsteps` substeps of the context from (x0, v0, E0) (float32 CPU
+tensors; x0, v0: n x 3, E0: n) with checkpoints every `every` substeps, and
+backpropagates dL/dx_T, dL/dv_T into dL/dx0, dL/dv0, dL/dE0 through
+mpm_adjoint_backward.  The context's initial volumes are computed from x0 on
+every forward and held fixed in the derivative.  Used by the gradient checks
+and the parameter fit; torch is imported only here.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+class Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sim, dt, nsteps, every, x0, v0, E0):
+        x = x0.detach().cpu().numpy().astype(np.float32)
+        v = v0.detach().cpu().numpy().astype(np.float32)
+        e = E0.detach().cpu().numpy().astype(np.float32)
+        m = getattr(sim, "rollout_mass")
+        sim.set_particles(x, v, m, e)
+        sim.init_volumes()
+        sim.adjoint_forward(dt, nsteps, every)
+        out = sim.particles()
+        ctx.sim = sim
+        return torch.from_numpy(out["x"].copy()), torch.from_numpy(out["v"].copy())
+
+    @staticmethod
+    def backward(ctx, gx, gv):
+        sim = ctx.sim
+        gx = None if gx is None else gx.detach().cpu().numpy().astype(np.float32)
+        gv = None if gv is None else gv.detach().cpu().numpy().astype(np.float32)
+        gx0, gv0, gE0 = sim.adjoint_backward(gx, gv)
+        return None, None, None, None, torch.from_numpy(gx0), torch.from_numpy(gv0), torch.from_numpy(gE0)
