@@ -93,9 +93,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device-to-device copy (calibrates rocprof byte counters)")
-    ap.add_argument("--workload", default="slab4m", choices=["slab4m", "batch64"],
+    ap.add_argument("--workload", default="slab4m", choices=["slab4m", "batch64", "adjoint16"],
                     help="slab4m: the headline 4M-particle slab; batch64: 64 independent scenes in one "
-                         "context against the same scenes run one after another")
+                         "context against the same scenes run one after another; adjoint16: forward and "
+                         "reverse mode over 16 scenes of 32,768 particles")
     ap.add_argument("--autotune", action="store_true", help="batch64: autotune the batch before timing")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -104,6 +105,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch64":
         return bench_batch(args)
+    if args.workload == "adjoint16":
+        return bench_adjoint(args)
 
     import numpy as np
     import mpm_amd as M
@@ -255,6 +258,79 @@ def bench_batch(args):
     }
     if tuned:
         out["autotune"] = tuned
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(args):
+    """16 snowballs of 32,768 particles on 64^3 (TEXTBOOK composition, domain
+    walls, dt 1e-4): `steps` substeps forward with checkpoints every 20, then
+    the reverse sweep for a synthetic loss on the final positions and
+    velocities.  A step is one forward plus one backward substep."""
+    import numpy as np
+    import mpm_amd as M
+    from mpm_amd import scenes
+
+    steps = args.steps
+    scs = scenes.scene_batch(seed=4, nscenes=16, n_range=(32768, 32768), split=32768)
+    ns = [s["x"].shape[0] for s in scs]
+    dt = scs[0]["dt"]
+
+    def params(sc):
+        prm = sc["params"]
+        return M.default_params(sc["dims"], sc["h"], compose=M.COMPOSE_TEXTBOOK, bc_mode=prm["bc_mode"], E=prm["E"],
+                                nu=prm["nu"], xi=prm["xi"], theta_c=prm["theta_c"], theta_s=prm["theta_s"])
+
+    b = M.Simulation([params(s) for s in scs], ns)
+    for i, s in enumerate(scs):
+        b.scene_set_particles(i, s["x"], s["v"], s["m"], s["E0"])
+    b.init_volumes()
+    n = b.n
+    rng = np.random.default_rng(9)
+    gx = rng.normal(size=(n, 3)).astype(np.float32)
+    gv = (rng.normal(size=(n, 3)) * 1e-3).astype(np.float32)
+    # warm-up: one short forward + backward
+    b.adjoint_forward(dt, 20, 20)
+    b.adjoint_backward(gx, gv)
+    b.sync()
+    t0 = time.perf_counter()
+    b.substeps(dt, steps)
+    b.sync()
+    plain_s = time.perf_counter() - t0
+    for i, s in enumerate(scs):
+        b.scene_set_particles(i, s["x"], s["v"], s["m"], s["E0"])
+    b.init_volumes()
+    b.sync()
+    t0 = time.perf_counter()
+    b.adjoint_forward(dt, steps, 20)
+    b.sync()
+    fwd_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gx0, gv0, gE0 = b.adjoint_backward(gx, gv)
+    b.sync()
+    bwd_s = time.perf_counter() - t0
+    b.close()
+    out = {
+        "metric": "fwd+bwd substeps/s, 16 x 32,768 particles on 64^3 (checkpoints every 20)",
+        "value": steps / (fwd_s + bwd_s),
+        "unit": "substeps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 20,
+        "ms_per_step": (fwd_s + bwd_s) / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "16 snowballs x 32,768 particles, 64^3, dt 1e-4, TEXTBOOK, checkpoints every 20",
+                   "particles": n, "scenes": len(scs)},
+        "forward_ms_per_substep": fwd_s / steps * 1e3,
+        "backward_ms_per_substep": bwd_s / steps * 1e3,
+        "plain_forward_ms_per_substep": plain_s / steps * 1e3,
+        "fwd_plus_bwd_over_fwd": (fwd_s + bwd_s) / fwd_s,
+        "particle_substeps_per_s_fwd_bwd": n * steps / (fwd_s + bwd_s),
+        "grad_finite": bool(np.isfinite(gx0).all() and np.isfinite(gv0).all() and np.isfinite(gE0).all()),
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -12,8 +12,9 @@
 //   x'    = clamp(x + dt v')
 // The reverse runs these backwards from the stored state of substep n:
 //   k_adj_grid_fwd  grid velocities of substep n from the re-run P2G sums
-//   k_adj_g2p       advection and G2P: node velocity adjoints (atomics) and
-//                   the position adjoint through the weights
+//   k_adj_g2p_block advection and G2P (mpm_fused.hip, on the block halos):
+//                   node velocity adjoints and the position adjoint through
+//                   the weights
 //   k_adj_grid      wall mask and v = P/m + dt g: adjoints of (m_i, P_i)
 //   k_adj_p2g       P2G (gathered), stress, hardening and the F update:
 //                   adjoints of x, v, B, F_E, F_P of substep n and of E0
@@ -44,31 +45,8 @@ __device__ __forceinline__ void st_m3(float* P, int64_t stride, int f, int p, co
     for (int r = 0; r < 3; ++r) P[(int64_t)(f + 3 * c + r) * stride + p] = m.c[c][r];
 }
 
-// 1-D weights and their derivatives d/dx on the stencil base-1 .. base+2
-struct AxisD {
-  int base;
-  float w[4], dw[4];
-};
-__device__ __forceinline__ float dweight(float u) {  // dN/du
-  const float a = fabsf(u);
-  if (a < 1.0f) return u * (1.5f * a - 2.0f);
-  if (a < 2.0f) {
-    const float t = 2.0f - a;
-    return (u < 0.0f ? 0.5f : -0.5f) * t * t;
-  }
-  return 0.0f;
-}
 __device__ __forceinline__ AxisD axis_d(float x, const Scene& S, int hi) {
-  const Axis a = axis_weights(x, S.h, S.inv_h, 1, hi);
-  AxisD r;
-  r.base = a.base;
-  const float q = div_rcp(x, S.h, S.inv_h);
-#pragma unroll
-  for (int d = 0; d < 4; ++d) {
-    r.w[d] = a.w[d];
-    r.dw[d] = dweight(q - (float)(a.base - 1 + d)) * S.inv_h;
-  }
-  return r;
+  return axis_weights_d(x, S.h, S.inv_h, 1, hi);
 }
 
 __device__ __forceinline__ M3 mzero() { return mdiag(0.0f); }
@@ -134,7 +112,7 @@ __device__ __forceinline__ bool svd_of(const M3& F, M3& Lb, float (&s)[3], M3& R
 // ------------------------------------------------------------------ kernels
 
 // grid velocities of the re-run substep from its P2G sums (gacc kept)
-__global__ void k_adj_grid_fwd(KArgs a) {
+__global__ void __launch_bounds__(TPB) k_adj_grid_fwd(KArgs a) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.nnodes) return;
   const float4 acc = a.gacc[g];
@@ -155,73 +133,8 @@ __global__ void k_adj_grid_fwd(KArgs a) {
   a.gvel[g] = out;
 }
 
-// advection and G2P adjoint.  St: stored state of substep n (slot order);
-// Gin: adjoints of substep n+1 (x, v, B at AX, AV, AB); Gout.x: position
-// adjoint (partial); gnode: node velocity adjoints (atomics).
-__global__ void k_adj_g2p(KArgs a, const float* St, const float* Gin, float* Gout, float4* gnode) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
-  const Scene S = a.scenes[scene_of_slot(a.scenes, a.nscenes, p)];
-  const int64_t sd = a.stride;
-  const F3 x = f3(fld(St, sd, FX, p), fld(St, sd, FX + 1, p), fld(St, sd, FX + 2, p));
-  AxisD ax[3];
-  ax[0] = axis_d(x.x, S, S.dims[0] - 3);
-  ax[1] = axis_d(x.y, S, S.dims[1] - 3);
-  ax[2] = axis_d(x.z, S, S.dims[2] - 3);
-  // forward G2P velocity for the advection clamp
-  F3 vn = f3(0.f, 0.f, 0.f);
-  for (int ia = 0; ia < 4; ++ia)
-    for (int ib = 0; ib < 4; ++ib)
-      for (int ic = 0; ic < 4; ++ic) {
-        const float w = ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic];
-        const float4 gv = a.gvel[node_index(ax[0].base - 1 + ia, ax[1].base - 1 + ib, ax[2].base - 1 + ic, S)];
-        vn = vn + f3(gv.x, gv.y, gv.z) * w;
-      }
-  const float xs[3] = {x.x, x.y, x.z}, vs[3] = {vn.x, vn.y, vn.z};
-  float gx[3], gv[3];
-  for (int k = 0; k < 3; ++k) {
-    const float xn = xs[k] + vs[k] * a.dt;
-    const float lo = 3.0f * S.h, hi = (float)(S.dims[k] - 3) * S.h;
-    const float mask = (xn > lo && xn < hi) ? 1.0f : 0.0f;
-    const float gxn = fld(Gin, sd, AX + k, p);
-    gx[k] = gxn * mask;
-    gv[k] = fld(Gin, sd, AV + k, p) + a.dt * gxn * mask;
-  }
-  const M3 gB = ld_m3(Gin, sd, AB, p);  // gB.c[c][r] = dL/dB(r, c)
-  for (int ia = 0; ia < 4; ++ia)
-    for (int ib = 0; ib < 4; ++ib)
-      for (int ic = 0; ic < 4; ++ic) {
-        const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib, k = ax[2].base - 1 + ic;
-        const float w = ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic];
-        const F3 dw = f3(ax[0].dw[ia] * ax[1].w[ib] * ax[2].w[ic], ax[0].w[ia] * ax[1].dw[ib] * ax[2].w[ic],
-                         ax[0].w[ia] * ax[1].w[ib] * ax[2].dw[ic]);
-        const int64_t g = node_index(i, j, k, S);
-        const float4 gv4 = a.gvel[g];
-        const F3 vi = f3(gv4.x, gv4.y, gv4.z);
-        const F3 d = f3((float)i * S.h - x.x, (float)j * S.h - x.y, (float)k * S.h - x.z);
-        // u = gv' + gB' d (what node velocity v_i multiplies, times w)
-        const F3 gBd = f3(gB.c[0][0] * d.x + gB.c[1][0] * d.y + gB.c[2][0] * d.z,
-                          gB.c[0][1] * d.x + gB.c[1][1] * d.y + gB.c[2][1] * d.z,
-                          gB.c[0][2] * d.x + gB.c[1][2] * d.y + gB.c[2][2] * d.z);
-        const F3 u = f3(gv[0], gv[1], gv[2]) + gBd;
-        const float su = dot(vi, u);
-        // gB'^T v_i
-        const F3 gBtv = f3(gB.c[0][0] * vi.x + gB.c[0][1] * vi.y + gB.c[0][2] * vi.z,
-                           gB.c[1][0] * vi.x + gB.c[1][1] * vi.y + gB.c[1][2] * vi.z,
-                           gB.c[2][0] * vi.x + gB.c[2][1] * vi.y + gB.c[2][2] * vi.z);
-        gx[0] += dw.x * su - w * gBtv.x;
-        gx[1] += dw.y * su - w * gBtv.y;
-        gx[2] += dw.z * su - w * gBtv.z;
-        float* gn = reinterpret_cast<float*>(gnode + g);
-        atomicAdd(gn + 1, w * u.x);
-        atomicAdd(gn + 2, w * u.y);
-        atomicAdd(gn + 3, w * u.z);
-      }
-  for (int k = 0; k < 3; ++k) Gout[(int64_t)(AX + k) * sd + p] = gx[k];
-}
-
 // wall mask and v = P/m + dt g: node adjoints (dL/dm, dL/dP) in place
-__global__ void k_adj_grid(KArgs a, float4* gnode) {
+__global__ void __launch_bounds__(TPB) k_adj_grid(KArgs a, float4* gnode) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.nnodes) return;
   const float4 acc = a.gacc[g];
@@ -259,7 +172,7 @@ __global__ void k_adj_grid(KArgs a, float4* gnode) {
 // P2G, stress, hardening and F-update adjoint for substep n.  St: state of
 // substep n; Gin: adjoints of substep n+1 (F_E, F_P); Gout: on entry the
 // partial position adjoint, on exit all adjoints of substep n; gE += dL/dE0.
-__global__ void k_adj_p2g(KArgs a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE) {
+__global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
   const Scene S = a.scenes[scene_of_slot(a.scenes, a.nscenes, p)];
@@ -387,30 +300,30 @@ __global__ void k_adj_p2g(KArgs a, const float* St, const float* Gin, float* Gou
 }
 
 // slot <-> caller order of `rows` consecutive rows
-__global__ void k_rows_to_slots(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows) {
+__global__ void __launch_bounds__(TPB) k_rows_to_slots(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int q = pid[p];
   for (int f = 0; f < rows; ++f) dst[(int64_t)f * stride + p] = src[(int64_t)f * stride + q];
 }
-__global__ void k_rows_to_caller(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows) {
+__global__ void __launch_bounds__(TPB) k_rows_to_caller(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int q = pid[p];
   for (int f = 0; f < rows; ++f) dst[(int64_t)f * stride + q] = src[(int64_t)f * stride + p];
 }
-__global__ void k_add_to_caller(float* dst, const float* src, const int* pid, int n) {
+__global__ void __launch_bounds__(TPB) k_add_to_caller(float* dst, const float* src, const int* pid, int n) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   dst[pid[p]] += src[p];
 }
 // n x comps interleaved <-> comps rows of `stride`
-__global__ void k_aos_to_rows(float* dst, const float* src, int n, int comps, int64_t stride) {
+__global__ void __launch_bounds__(TPB) k_aos_to_rows(float* dst, const float* src, int n, int comps, int64_t stride) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   for (int c = 0; c < comps; ++c) dst[(int64_t)c * stride + p] = src[(int64_t)p * comps + c];
 }
-__global__ void k_rows_to_aos(float* dst, const float* src, int n, int comps, int64_t stride) {
+__global__ void __launch_bounds__(TPB) k_rows_to_aos(float* dst, const float* src, int n, int comps, int64_t stride) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   for (int c = 0; c < comps; ++c) dst[(int64_t)p * comps + c] = src[(int64_t)c * stride + p];
@@ -419,9 +332,6 @@ __global__ void k_rows_to_aos(float* dst, const float* src, int n, int comps, in
 }  // namespace
 
 void launch_adj_grid_fwd(const KArgs& a, hipStream_t s) { k_adj_grid_fwd<<<nblk(a.nnodes), TPB, 0, s>>>(a); }
-void launch_adj_g2p(const KArgs& a, const float* St, const float* Gin, float* Gout, float4* gnode, hipStream_t s) {
-  if (a.n) k_adj_g2p<<<nblk(a.n), TPB, 0, s>>>(a, St, Gin, Gout, gnode);
-}
 void launch_adj_grid(const KArgs& a, float4* gnode, hipStream_t s) {
   k_adj_grid<<<nblk(a.nnodes), TPB, 0, s>>>(a, gnode);
 }

@@ -1098,7 +1098,8 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
       KArgs a = make_args(ctx, dt);
       launch_adj_grid_fwd(a, s);
       launch_clear(gnode, ctx->nnodes, s);
-      launch_adj_g2p(a, St, Gs, Gs2, gnode, s);
+      ba.a.P = ctx->P;
+      launch_adj_g2p_block(ba, St, Gs, Gs2, gnode, s);
       launch_adj_grid(a, gnode, s);
       launch_adj_p2g(a, St, Gs, Gs2, gnode, gEs, s);
       std::swap(Gs, Gs2);

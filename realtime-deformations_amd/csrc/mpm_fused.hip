@@ -373,6 +373,157 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 1) k_blo
   }
 }
 
+// Reverse of advection + G2P for the blocks of the current binning (see
+// launch_adj_g2p_block).  The velocities are those of the re-run substep
+// (gvel); the stage order and notation are those of mpm_adjoint.hip.
+template <int BATCHED>
+__global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float* __restrict__ St,
+                                                       const float* __restrict__ Gin, float* __restrict__ Gout,
+                                                       float4* __restrict__ gnode) {
+  __shared__ float vh[3 * PLANE];
+  __shared__ double gh[3 * PLANE];
+  const KArgs& a = ba.a;
+  const int t = threadIdx.x;
+  const int64_t sd = a.stride;
+  const int nblocks = *ba.nblk;
+  for (int s = blockIdx.x; s < nblocks; s += gridDim.x) {
+    const int4 e = ba.blist[s];
+    const int scene = e.w, start = e.y, end = e.z;
+    const Scene S = BATCHED ? a.scenes[scene] : a.sc;
+    const int blk = (int)(e.x - S.blk0);
+    const int bk = blk % S.nb[2];
+    const int tt = blk / S.nb[2];
+    const int bj = tt % S.nb[1];
+    const int bi = tt / S.nb[1];
+    const int o[3] = {4 * bi - 2, 4 * bj - 2, 4 * bk - 2};
+    for (int q = t; q < HN * HN * HN; q += TPB) {
+      const int hx = q / (HN * HN), hy = (q / HN) % HN, hz = q % HN;
+      const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
+      float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2])
+        g = a.gvel[node_index(gi, gj, gk, S)];
+      const int l = hx * SX + hy * SY + hz;
+      vh[l] = g.x;
+      vh[PLANE + l] = g.y;
+      vh[2 * PLANE + l] = g.z;
+    }
+    for (int q = t; q < 3 * PLANE; q += TPB) gh[q] = 0.0;
+    __syncthreads();
+    for (int p = start + t; p < end; p += TPB) {
+      const F3 x = f3(St[(int64_t)FX * sd + p], St[(int64_t)(FX + 1) * sd + p], St[(int64_t)(FX + 2) * sd + p]);
+      AxisD ax[3];
+      ax[0] = axis_weights_d(x.x, S.h, S.inv_h, 1, S.dims[0] - 3);
+      ax[1] = axis_weights_d(x.y, S.h, S.inv_h, 1, S.dims[1] - 3);
+      ax[2] = axis_weights_d(x.z, S.h, S.inv_h, 1, S.dims[2] - 3);
+      int lb[3];
+      bool fits = true;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        lb[d] = ax[d].base - 1 - o[d];
+        fits = fits && lb[d] >= 0 && lb[d] + 3 < HN;
+      }
+      const int q0 = lb[0] * SX + lb[1] * SY + lb[2];
+      auto fetch = [&](int ia, int ib, int ic) {
+        if (fits) {
+          const int q = q0 + ia * SX + ib * SY + ic;
+          return f3(vh[q], vh[PLANE + q], vh[2 * PLANE + q]);
+        }
+        const float4 g = a.gvel[node_index(ax[0].base - 1 + ia, ax[1].base - 1 + ib, ax[2].base - 1 + ic, S)];
+        return f3(g.x, g.y, g.z);
+      };
+      // forward G2P velocity, for the advection clamp
+      F3 vn = f3(0.f, 0.f, 0.f);
+#pragma unroll
+      for (int ia = 0; ia < 4; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+          for (int ic = 0; ic < 4; ++ic) vn = vn + fetch(ia, ib, ic) * (ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic]);
+      // refetch the node velocities below instead of keeping all 64 live
+      __asm__ __volatile__("" ::: "memory");
+      const float xs[3] = {x.x, x.y, x.z}, vs[3] = {vn.x, vn.y, vn.z};
+      float gx[3], gvv[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float xn = xs[k] + vs[k] * a.dt;
+        const float lo = 3.0f * S.h, hi = (float)(S.dims[k] - 3) * S.h;
+        const float mask = (xn > lo && xn < hi) ? 1.0f : 0.0f;
+        const float gxn = Gin[(int64_t)(FX + k) * sd + p];
+        gx[k] = gxn * mask;
+        gvv[k] = Gin[(int64_t)(FV + k) * sd + p] + a.dt * gxn * mask;
+      }
+      M3 gB;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) gB.c[c][r] = Gin[(int64_t)(FB + 3 * c + r) * sd + p];
+      // one node: its velocity vi, weight w and weight gradient dw; returns
+      // the node adjoint w (gv' + gB' d) and adds the position terms to gx
+      auto node = [&](int ia, int ib, int ic, F3 vi) {
+        const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib, k = ax[2].base - 1 + ic;
+        const float w = ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic];
+        const F3 dw = f3(ax[0].dw[ia] * ax[1].w[ib] * ax[2].w[ic], ax[0].w[ia] * ax[1].dw[ib] * ax[2].w[ic],
+                         ax[0].w[ia] * ax[1].w[ib] * ax[2].dw[ic]);
+        const F3 d = f3((float)i * S.h - x.x, (float)j * S.h - x.y, (float)k * S.h - x.z);
+        const F3 u = f3(gvv[0] + gB.c[0][0] * d.x + gB.c[1][0] * d.y + gB.c[2][0] * d.z,
+                        gvv[1] + gB.c[0][1] * d.x + gB.c[1][1] * d.y + gB.c[2][1] * d.z,
+                        gvv[2] + gB.c[0][2] * d.x + gB.c[1][2] * d.y + gB.c[2][2] * d.z);
+        const float su = dot(vi, u);
+        gx[0] += dw.x * su - w * (gB.c[0][0] * vi.x + gB.c[0][1] * vi.y + gB.c[0][2] * vi.z);
+        gx[1] += dw.y * su - w * (gB.c[1][0] * vi.x + gB.c[1][1] * vi.y + gB.c[1][2] * vi.z);
+        gx[2] += dw.z * su - w * (gB.c[2][0] * vi.x + gB.c[2][1] * vi.y + gB.c[2][2] * vi.z);
+        return u * w;
+      };
+      if (fits) {
+#pragma unroll
+        for (int ia = 0; ia < 4; ++ia) {
+          __asm__ __volatile__("" ::: "memory");  // at most one x slab of nodes in flight
+#pragma unroll
+          for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+            for (int ic = 0; ic < 4; ++ic) {
+              const int q = q0 + ia * SX + ib * SY + ic;
+              const F3 wu = node(ia, ib, ic, f3(vh[q], vh[PLANE + q], vh[2 * PLANE + q]));
+              atomicAdd(&gh[q], (double)wu.x);
+              atomicAdd(&gh[PLANE + q], (double)wu.y);
+              atomicAdd(&gh[2 * PLANE + q], (double)wu.z);
+            }
+        }
+      } else {
+        for (int ia = 0; ia < 4; ++ia)
+          for (int ib = 0; ib < 4; ++ib)
+            for (int ic = 0; ic < 4; ++ic) {
+              const int64_t g = node_index(ax[0].base - 1 + ia, ax[1].base - 1 + ib, ax[2].base - 1 + ic, S);
+              const float4 gv4 = a.gvel[g];
+              const F3 wu = node(ia, ib, ic, f3(gv4.x, gv4.y, gv4.z));
+              float* gn = reinterpret_cast<float*>(gnode + g);
+              atomicAdd(gn + 1, wu.x);
+              atomicAdd(gn + 2, wu.y);
+              atomicAdd(gn + 3, wu.z);
+            }
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) Gout[(int64_t)(FX + k) * sd + p] = gx[k];
+    }
+    __syncthreads();
+    for (int q = t; q < HN * HN * HN; q += TPB) {
+      const int hx = q / (HN * HN), hy = (q / HN) % HN, hz = q % HN;
+      const int l = hx * SX + hy * SY + hz;
+      const float u0 = (float)gh[l], u1 = (float)gh[PLANE + l], u2 = (float)gh[2 * PLANE + l];
+      if (u0 != 0.0f || u1 != 0.0f || u2 != 0.0f) {
+        const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
+        if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2]) {
+          float* gn = reinterpret_cast<float*>(gnode + node_index(gi, gj, gk, S));
+          atomicAdd(gn + 1, u0);
+          atomicAdd(gn + 2, u1);
+          atomicAdd(gn + 3, u2);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // v = p / m + dt g on the nodes of active blocks, boundary conditions, and
 // the accumulator reset; blocks active last substep but not now are zeroed.
 template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(BlockArgs ba) {
@@ -454,6 +605,14 @@ void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s) {
     launch_block_pass_t<0>(ba, mode, s);
   else
     launch_block_pass_t<1>(ba, mode, s);
+}
+
+void launch_adj_g2p_block(const BlockArgs& ba, const float* St, const float* Gin, float* Gout, float4* gnode,
+                          hipStream_t s) {
+  if (ba.a.nscenes == 1)
+    k_adj_g2p_block<0><<<ba.grid, TPB, 0, s>>>(ba, St, Gin, Gout, gnode);
+  else
+    k_adj_g2p_block<1><<<ba.grid, TPB, 0, s>>>(ba, St, Gin, Gout, gnode);
 }
 
 void launch_grid_active(const BlockArgs& ba, hipStream_t s) {

@@ -196,6 +196,34 @@ MPM_HD Axis axis_weights(float x, float h, float inv_h, int lo, int hi) {
   return a;
 }
 
+// dN/du of the cubic B-spline
+MPM_HD float dweightN(float u) {
+  const float a = fabsf(u);
+  if (a < 1.0f) return u * (1.5f * a - 2.0f);
+  if (a < 2.0f) {
+    const float t = 2.0f - a;
+    return (u < 0.0f ? 0.5f : -0.5f) * t * t;
+  }
+  return 0.0f;
+}
+// Stencil weights with their derivatives d/dx (reverse mode).
+struct AxisD {
+  int base;
+  float w[4], dw[4];
+};
+MPM_HD AxisD axis_weights_d(float x, float h, float inv_h, int lo, int hi) {
+  const Axis a = axis_weights(x, h, inv_h, lo, hi);
+  AxisD r;
+  r.base = a.base;
+  const float q = div_rcp(x, h, inv_h);
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    r.w[d] = a.w[d];
+    r.dw[d] = dweightN(q - (float)(a.base - 1 + d)) * inv_h;
+  }
+  return r;
+}
+
 // ------------------------------------------------ Eigen-convention Jacobi
 template <typename T> struct Rot {
   T c, s;
