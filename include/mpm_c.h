@@ -162,6 +162,68 @@ int mpm_scene_get_grid(mpm_ctx* ctx, int32_t scene, float* m, float* vel, float*
  * others); clears the flags. */
 int mpm_scene_status(mpm_ctx* ctx, int32_t* status);
 
+/* ---- device scene generators.  Seeded particle placement written on the
+ * GPU into a scene's particle slots (caller order = generation order), with
+ * F_E = F_P = I, B = 0, V0 = 0 as after mpm_set_particles; follow with
+ * mpm_init_volumes.  Counter-based randomness (Philox4x64-10 keyed by seed):
+ * the same seed gives the same particles on any launch shape.
+ *   MPM_GEN_LATTICE replaces initializeParticles
+ *     (realtime-deformations/material_point_method.cpp:18-63): eight
+ *     quarter-offset candidates per cell of the cube of half width
+ *     int(radius / h) cells around ivec3(centre / h), jittered uniformly in a
+ *     ball of 1/4 cell (the reference draws it from rand(), utils.h:114-127),
+ *     kept within `radius` (the reference's width) of `centre`, filled from the
+ *     back of the array.  *candidates returns the kept count; fewer than the
+ *     scene's particles is MPM_ERR_ARG (the reference leaves those particles
+ *     unplaced).
+ *   MPM_GEN_BALL (no reference counterpart): n points uniform in a ball.
+ *   MPM_GEN_CELLS (no reference counterpart): a box of cells with
+ *     U{ppc_lo..ppc_hi} particles each, adjusted by +-1 steps in a seeded cell
+ *     order to exactly the scene's count, uniform inside each cell.
+ * E0 is uniform in [E_lo, E_hi) (E_lo == E_hi: constant).  A particle whose
+ * stencil leaves the grid is MPM_ERR_BOUNDS and leaves the scene unset. ---- */
+enum { MPM_GEN_BALL = 0, MPM_GEN_CELLS = 1, MPM_GEN_LATTICE = 2 };
+typedef struct mpm_generator {
+  int32_t kind;
+  int32_t reserved;
+  uint64_t seed;
+  float centre[3];     /* BALL: centre; LATTICE: origin */
+  float radius;        /* BALL: radius; LATTICE: width */
+  int32_t cell_lo[3];  /* CELLS: first cell (i, j, k) of the box */
+  int32_t cells[3];    /* CELLS: box extent in cells */
+  int32_t ppc_lo, ppc_hi;
+  float velocity[3];
+  float mass;          /* per particle */
+  float E_lo, E_hi;
+} mpm_generator;
+int mpm_scene_generate(mpm_ctx* ctx, int32_t scene, const mpm_generator* g, int64_t* candidates);
+int mpm_generate(mpm_ctx* ctx, const mpm_generator* g, int64_t* candidates);
+
+/* ---- readback and snapshots ---------------------------------------------
+ * m and E0 of one scene (caller order; NULL skips). */
+int mpm_scene_get_material(mpm_ctx* ctx, int32_t scene, float* m, float* E0);
+/* Positions only, all particles in caller order (n x 3): the viewer's read
+ * of getParticles() each frame (realtime-deformations/main.cpp:257-271).
+ * Goes through a context-owned device staging buffer and pinned host mirror
+ * (allocated on first use, 24 bytes per particle).  mpm_get_positions copies
+ * into x and synchronises.  mpm_positions_begin enqueues the gather on the
+ * context stream and the copy on a second stream, so it overlaps substeps
+ * issued after it; mpm_positions_wait blocks until the copy lands and points
+ * *x at the pinned mirror, valid until the next mpm_positions_begin. */
+int mpm_get_positions(mpm_ctx* ctx, float* x);
+int mpm_positions_begin(mpm_ctx* ctx);
+int mpm_positions_wait(mpm_ctx* ctx, const float** x);
+/* Snapshot file: every scene's parameters, colliders and full particle state
+ * (x, v, B, F_E, F_P, m, V0, E0 in caller order) plus a caller step counter;
+ * the layout is documented in mpm_io.hip and read by mpm_amd.snapshot.
+ * mpm_snapshot_load needs a context with the same scenes (particle counts and
+ * parameters) and replaces its state, clearing the status flags;
+ * mpm_create_from_snapshot creates that context.  Resume = create from the
+ * snapshot and continue with mpm_substeps. */
+int mpm_snapshot_save(mpm_ctx* ctx, const char* path, int64_t step);
+int mpm_snapshot_load(mpm_ctx* ctx, const char* path, int64_t* step);
+int mpm_create_from_snapshot(const char* path, mpm_ctx** out, int64_t* step);
+
 /* ---- launch tuning (no reference counterpart).  resort_every: substeps
  * between re-binnings; block_grid: workgroups of the persistent block pass.
  * mpm_autotune times `steps` substeps per candidate (resort_every in

@@ -85,6 +85,12 @@ def lib():
         L.orc_collider_inverse_f.argtypes = [_F, _F]
         L.orc_collide_f.argtypes = [_F, C.c_int, _F, _F, _F]
         L.orc_domain_walls_f.argtypes = [C.c_int] * 6 + [_F, _F]
+        _U64 = C.POINTER(C.c_uint64)
+        _I = C.POINTER(C.c_int)
+        L.orc_philox4x64.argtypes = [_U64, _U64, _U64]
+        L.orc_generate.restype = C.c_int64
+        L.orc_generate.argtypes = [C.c_int, C.c_uint64, _F, C.c_float, _I, _I, C.c_int, C.c_int, C.c_float,
+                                   C.c_float, C.c_float, C.c_int, _F, _F]
         _lib = L
     return _lib
 
@@ -282,3 +288,44 @@ def bin_reference(x, h, dims):
     slot = np.empty(n, np.int64)
     slot[starts[block_of] + pos] = perm
     return skey, perm.astype(np.uint32), slot
+
+
+# ------------------------------------------------------------ generators
+# Seeded scene generators (BALL, CELLS, LATTICE); LATTICE follows
+# initializeParticles (realtime-deformations/material_point_method.cpp:18-63)
+# with a Philox jitter in place of rand().
+GEN_BALL, GEN_CELLS, GEN_LATTICE = 0, 1, 2
+
+
+def philox4x64(ctr, key):
+    """Philox4x64-10 block for counter ctr (4 words) and key (2 words)."""
+    c = np.ascontiguousarray(ctr, np.uint64)
+    k = np.ascontiguousarray(key, np.uint64)
+    out = np.zeros(4, np.uint64)
+    u = C.POINTER(C.c_uint64)
+    lib().orc_philox4x64(c.ctypes.data_as(u), k.ctypes.data_as(u), out.ctypes.data_as(u))
+    return out
+
+
+def generate(kind, n, h, seed=0, centre=(0.5, 0.5, 0.5), radius=0.1, cell_lo=(0, 0, 0), cells=(1, 1, 1),
+             ppc=(4, 12), E=(1.4e5, 1.4e5)):
+    """-> (x (n,3) f32, E0 (n,) f32, candidates kept).  Raises ValueError when
+    the request cannot be met (returned count then says how many fit)."""
+    x = np.zeros((n, 3), np.float32)
+    E0 = np.zeros(n, np.float32)
+    ce = np.ascontiguousarray(centre, np.float32)
+    lo = np.ascontiguousarray(cell_lo, np.int32)
+    ex = np.ascontiguousarray(cells, np.int32)
+    ip = C.POINTER(C.c_int)
+    r = lib().orc_generate(int(kind), int(seed), fptr(ce), float(radius), lo.ctypes.data_as(ip),
+                           ex.ctypes.data_as(ip), int(ppc[0]), int(ppc[1]), float(E[0]), float(E[1]), float(h), int(n),
+                           fptr(x), fptr(E0))
+    if r < 0:
+        raise ValueError(f"generator request cannot be met (kept {-1 - r} candidates)" if kind == GEN_LATTICE
+                         else "generator request cannot be met")
+    return x, E0, int(r)
+
+
+def lattice_candidates(h, seed, centre, radius):
+    """Number of LATTICE candidates the rule keeps (an n = 0 request)."""
+    return generate(GEN_LATTICE, 0, h, seed=seed, centre=centre, radius=radius)[2]

@@ -29,6 +29,8 @@ SOURCES = [
     "mpm_bin.hip",
     "mpm_capi.hip",
     "mpm_adjoint.hip",
+    "mpm_gen.hip",
+    "mpm_io.hip",
 ]
 FACADE = os.path.join(PKG, "facade")
 DEMO = os.path.join(LIBDIR, "facade_demo")

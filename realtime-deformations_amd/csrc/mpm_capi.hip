@@ -247,6 +247,45 @@ void host_collider_inverse(const float t[3], const float q[4], float inv[4][4]) 
 }
 }  // namespace mpm
 
+namespace mpm {
+int upload_field(mpm_ctx* ctx, int scene, int field, const float* host, int comps) {
+  const Scene& S = ctx->scenes[scene];
+  if (!host || S.np == 0) return MPM_OK;
+  float* tmp = nullptr;
+  const size_t bytes = sizeof(float) * (size_t)S.np * comps;
+  MPM_HIP(hipMalloc(&tmp, bytes));
+  MPM_HIP(hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  launch_scatter_field(ctx->P + (int64_t)field * ctx->stride, ctx->stride, tmp, ctx->pid, S.p0, S.np, comps,
+                       ctx->stream);
+  MPM_HIP(hipStreamSynchronize(ctx->stream));
+  MPM_HIP(hipFree(tmp));
+  return MPM_OK;
+}
+int download_field(mpm_ctx* ctx, int scene, int field, float* host, int comps) {
+  const Scene& S = ctx->scenes[scene];
+  if (!host || S.np == 0) return MPM_OK;
+  float* tmp = nullptr;
+  const size_t bytes = sizeof(float) * (size_t)S.np * comps;
+  MPM_HIP(hipMalloc(&tmp, bytes));
+  launch_gather_field(tmp, ctx->P + (int64_t)field * ctx->stride, ctx->stride, ctx->pid, S.p0, S.np, comps,
+                      ctx->stream);
+  MPM_HIP(hipMemcpyAsync(host, tmp, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  MPM_HIP(hipStreamSynchronize(ctx->stream));
+  MPM_HIP(hipFree(tmp));
+  return MPM_OK;
+}
+int check_scene_positions(mpm_ctx* ctx, int scene, const float* x) { return check_positions(ctx, scene, x); }
+int reset_scene_order(mpm_ctx* ctx, int scene) {
+  const Scene& S = ctx->scenes[scene];
+  MPM_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<int> ids((size_t)S.np);
+  for (int i = 0; i < S.np; ++i) ids[i] = S.p0 + i;
+  if (S.np) MPM_HIP(hipMemcpy(ctx->pid + S.p0, ids.data(), sizeof(int) * S.np, hipMemcpyHostToDevice));
+  ctx->since_sort = -1;
+  return MPM_OK;
+}
+}  // namespace mpm
+
 extern "C" {
 
 void mpm_default_params(mpm_params* p, int32_t ni, int32_t nj, int32_t nk, float h) {
@@ -287,6 +326,8 @@ int mpm_create_batch(const mpm_params* params, const int32_t* n_particles, int32
   if (!ctx) return MPM_ERR_NOMEM;
   *out = ctx;
   ctx->prm = params[0];
+  ctx->sprm.assign(params, params + nscenes);
+  ctx->hcoll.assign(nscenes, {});
   ctx->nscenes = nscenes;
   ctx->scenes.resize(nscenes);
   ctx->scoll.assign(nscenes, nullptr);
@@ -393,6 +434,12 @@ int mpm_destroy(mpm_ctx* ctx) {
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->ev_region)
     if (e) (void)hipEventDestroy(e);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+  for (hipEvent_t e : {ctx->ev_gathered, ctx->ev_copied})
+    if (e) (void)hipEventDestroy(e);
+  (void)hipFree(ctx->pos_stage);
+  if (ctx->pos_host) (void)hipHostFree(ctx->pos_host);
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MPM_OK;
@@ -402,32 +449,6 @@ const char* mpm_last_error(const mpm_ctx* ctx) { return ctx ? ctx->err.c_str() :
 int32_t mpm_num_particles(const mpm_ctx* ctx) { return ctx ? ctx->n : 0; }
 int32_t mpm_num_scenes(const mpm_ctx* ctx) { return ctx ? ctx->nscenes : 0; }
 
-static int upload_fields(mpm_ctx* ctx, int scene, int field, const float* host, int comps) {
-  const Scene& S = ctx->scenes[scene];
-  if (!host || S.np == 0) return MPM_OK;
-  float* tmp = nullptr;
-  const size_t bytes = sizeof(float) * (size_t)S.np * comps;
-  MPM_HIP(hipMalloc(&tmp, bytes));
-  MPM_HIP(hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, ctx->stream));
-  launch_scatter_field(ctx->P + (int64_t)field * ctx->stride, ctx->stride, tmp, ctx->pid, S.p0, S.np, comps,
-                       ctx->stream);
-  MPM_HIP(hipStreamSynchronize(ctx->stream));
-  MPM_HIP(hipFree(tmp));
-  return MPM_OK;
-}
-static int download_fields(mpm_ctx* ctx, int scene, int field, float* host, int comps) {
-  const Scene& S = ctx->scenes[scene];
-  if (!host || S.np == 0) return MPM_OK;
-  float* tmp = nullptr;
-  const size_t bytes = sizeof(float) * (size_t)S.np * comps;
-  MPM_HIP(hipMalloc(&tmp, bytes));
-  launch_gather_field(tmp, ctx->P + (int64_t)field * ctx->stride, ctx->stride, ctx->pid, S.p0, S.np, comps,
-                      ctx->stream);
-  MPM_HIP(hipMemcpyAsync(host, tmp, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  MPM_HIP(hipStreamSynchronize(ctx->stream));
-  MPM_HIP(hipFree(tmp));
-  return MPM_OK;
-}
 
 int mpm_scene_set_particles(mpm_ctx* ctx, int32_t scene, const float* x, const float* v, const float* m,
                             const float* E0) {
@@ -440,23 +461,19 @@ int mpm_scene_set_particles(mpm_ctx* ctx, int32_t scene, const float* x, const f
   }
   if (int rc = check_positions(ctx, scene, x)) return rc;
   // caller order again inside this scene's slots; every scene re-bins next
-  MPM_HIP(hipStreamSynchronize(ctx->stream));
-  std::vector<int> ids((size_t)S.np);
-  for (int i = 0; i < S.np; ++i) ids[i] = S.p0 + i;
-  if (S.np) MPM_HIP(hipMemcpy(ctx->pid + S.p0, ids.data(), sizeof(int) * S.np, hipMemcpyHostToDevice));
-  ctx->since_sort = -1;
   int rc;
-  if ((rc = upload_fields(ctx, scene, FX, x, 3)) || (rc = upload_fields(ctx, scene, FV, v, 3)) ||
-      (rc = upload_fields(ctx, scene, FM, m, 1)))
+  if ((rc = reset_scene_order(ctx, scene))) return rc;
+  if ((rc = upload_field(ctx, scene, FX, x, 3)) || (rc = upload_field(ctx, scene, FV, v, 3)) ||
+      (rc = upload_field(ctx, scene, FM, m, 1)))
     return rc;
   std::vector<float> ident((size_t)S.np * 9, 0.0f), zeros((size_t)S.np * 9, 0.0f), e((size_t)S.np);
   for (int p = 0; p < S.np; ++p) {
     ident[9 * p + 0] = ident[9 * p + 4] = ident[9 * p + 8] = 1.0f;
     e[p] = E0 ? E0[p] : S.E;
   }
-  if ((rc = upload_fields(ctx, scene, FB, zeros.data(), 9)) || (rc = upload_fields(ctx, scene, FFE, ident.data(), 9)) ||
-      (rc = upload_fields(ctx, scene, FFP, ident.data(), 9)) || (rc = upload_fields(ctx, scene, FVOL, zeros.data(), 1)) ||
-      (rc = upload_fields(ctx, scene, FE0, e.data(), 1)))
+  if ((rc = upload_field(ctx, scene, FB, zeros.data(), 9)) || (rc = upload_field(ctx, scene, FFE, ident.data(), 9)) ||
+      (rc = upload_field(ctx, scene, FFP, ident.data(), 9)) || (rc = upload_field(ctx, scene, FVOL, zeros.data(), 1)) ||
+      (rc = upload_field(ctx, scene, FE0, e.data(), 1)))
     return rc;
   ctx->scene_set[scene] = 1;
   ctx->particles_set = true;
@@ -480,8 +497,8 @@ int mpm_scene_set_state(mpm_ctx* ctx, int32_t scene, const float* B, const float
   if (check_ctx(ctx)) return MPM_ERR_ARG;
   if (int rc = valid_scene(ctx, scene)) return rc;
   int rc;
-  if ((rc = upload_fields(ctx, scene, FB, B, 9)) || (rc = upload_fields(ctx, scene, FFE, FE, 9)) ||
-      (rc = upload_fields(ctx, scene, FFP, FP, 9)) || (rc = upload_fields(ctx, scene, FVOL, V0, 1)))
+  if ((rc = upload_field(ctx, scene, FB, B, 9)) || (rc = upload_field(ctx, scene, FFE, FE, 9)) ||
+      (rc = upload_field(ctx, scene, FFP, FP, 9)) || (rc = upload_field(ctx, scene, FVOL, V0, 1)))
     return rc;
   return MPM_OK;
 }
@@ -518,6 +535,7 @@ int mpm_scene_set_colliders(mpm_ctx* ctx, int32_t scene, const mpm_collider* c, 
   }
   ctx->scenes[scene].coll = dc;
   ctx->scenes[scene].ncoll = k;
+  ctx->hcoll[scene].assign(c, c + k);
   return upload_scene_table(ctx);
 }
 
@@ -912,9 +930,9 @@ int mpm_scene_get_particles(mpm_ctx* ctx, int32_t scene, float* x, float* v, flo
   if (check_ctx(ctx)) return MPM_ERR_ARG;
   if (int rc = valid_scene(ctx, scene)) return rc;
   int rc;
-  if ((rc = download_fields(ctx, scene, FX, x, 3)) || (rc = download_fields(ctx, scene, FV, v, 3)) ||
-      (rc = download_fields(ctx, scene, FB, B, 9)) || (rc = download_fields(ctx, scene, FFE, FE, 9)) ||
-      (rc = download_fields(ctx, scene, FFP, FP, 9)) || (rc = download_fields(ctx, scene, FVOL, V0, 1)))
+  if ((rc = download_field(ctx, scene, FX, x, 3)) || (rc = download_field(ctx, scene, FV, v, 3)) ||
+      (rc = download_field(ctx, scene, FB, B, 9)) || (rc = download_field(ctx, scene, FFE, FE, 9)) ||
+      (rc = download_field(ctx, scene, FFP, FP, 9)) || (rc = download_field(ctx, scene, FVOL, V0, 1)))
     return rc;
   return MPM_OK;
 }

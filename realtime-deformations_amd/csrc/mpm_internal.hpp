@@ -90,6 +90,8 @@ __device__ __forceinline__ int scene_of_slot(const Scene* scenes, int nscenes, i
 struct mpm_ctx {
   mpm_params prm{};            // scene 0's parameters (compose, resort_every for all)
   int nscenes = 1;
+  std::vector<mpm_params> sprm;                    // each scene's parameters as created
+  std::vector<std::vector<mpm_collider>> hcoll;    // each scene's colliders as set
   std::vector<mpm::Scene> scenes;          // host copy of the scene table
   std::vector<mpm::DevCollider*> scoll;    // per-scene collider arrays (device)
   mpm::Scene* d_scenes = nullptr;          // device scene table
@@ -146,6 +148,12 @@ struct mpm_ctx {
   int adj_steps = 0, adj_every = 0;
   std::vector<float*> adj_ckpt;
   float* adj_const = nullptr;
+  // positions readback: device staging, pinned host mirror, copy stream
+  float* pos_stage = nullptr;
+  float* pos_host = nullptr;     // pinned
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_gathered = nullptr, ev_copied = nullptr;
+  bool pos_pending = false;
 };
 
 namespace mpm {
@@ -176,4 +184,12 @@ void host_collider_inverse(const float t[3], const float q[4], float inv[4][4]);
 void launch_count_active(const KArgs& a, int64_t* out, hipStream_t s);
 // batched initial volumes: density gathered from the node masses in gvel.w
 void launch_volumes(const KArgs& a, hipStream_t s);
+// one field of one scene between the caller-ordered host array and the
+// particle rows (synchronous); NULL host skips
+int upload_field(mpm_ctx* ctx, int scene, int field, const float* host, int comps);
+int download_field(mpm_ctx* ctx, int scene, int field, float* host, int comps);
+// every particle's stencil inside its scene's grid (MPM_ERR_BOUNDS otherwise)
+int check_scene_positions(mpm_ctx* ctx, int scene, const float* x);
+// caller order again inside a scene's slots (slot p0 + i holds particle i)
+int reset_scene_order(mpm_ctx* ctx, int scene);
 }  // namespace mpm

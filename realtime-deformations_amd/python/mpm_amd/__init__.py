@@ -57,6 +57,27 @@ class Collider(C.Structure):
     ]
 
 
+class Generator(C.Structure):
+    """mpm_generator: seeded device placement of a scene's particles."""
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("reserved", C.c_int32),
+        ("seed", C.c_uint64),
+        ("centre", C.c_float * 3),
+        ("radius", C.c_float),
+        ("cell_lo", C.c_int32 * 3),
+        ("cells", C.c_int32 * 3),
+        ("ppc_lo", C.c_int32),
+        ("ppc_hi", C.c_int32),
+        ("velocity", C.c_float * 3),
+        ("mass", C.c_float),
+        ("E_lo", C.c_float),
+        ("E_hi", C.c_float),
+    ]
+
+
+GEN_BALL, GEN_CELLS, GEN_LATTICE = 0, 1, 2
+
 _F = C.POINTER(C.c_float)
 _lib = None
 
@@ -69,6 +90,8 @@ EXPORTS = [
     "mpm_create_batch", "mpm_num_scenes", "mpm_scene_set_particles", "mpm_scene_set_state",
     "mpm_scene_set_colliders", "mpm_scene_get_particles", "mpm_scene_get_grid", "mpm_scene_status",
     "mpm_set_tuning", "mpm_get_tuning", "mpm_autotune", "mpm_adjoint_forward", "mpm_adjoint_backward",
+    "mpm_scene_generate", "mpm_generate", "mpm_scene_get_material", "mpm_get_positions", "mpm_positions_begin",
+    "mpm_positions_wait", "mpm_snapshot_save", "mpm_snapshot_load", "mpm_create_from_snapshot",
 ]
 KERNELS = {"p2g": 0, "grid": 1, "g2p": 2, "bin": 3, "fused": 4}
 NKERNELS = 8
@@ -116,6 +139,15 @@ def lib():
         L.mpm_count_active.argtypes = [vp, C.POINTER(C.c_int64)]
         L.mpm_debug_bin.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]
         L.mpm_copy_probe.argtypes = [vp, C.c_int64]
+        L.mpm_scene_generate.argtypes = [vp, C.c_int32, C.POINTER(Generator), C.POINTER(C.c_int64)]
+        L.mpm_generate.argtypes = [vp, C.POINTER(Generator), C.POINTER(C.c_int64)]
+        L.mpm_scene_get_material.argtypes = [vp, C.c_int32, _F, _F]
+        L.mpm_get_positions.argtypes = [vp, _F]
+        L.mpm_positions_begin.argtypes = [vp]
+        L.mpm_positions_wait.argtypes = [vp, C.POINTER(_F)]
+        L.mpm_snapshot_save.argtypes = [vp, C.c_char_p, C.c_int64]
+        L.mpm_snapshot_load.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
+        L.mpm_create_from_snapshot.argtypes = [C.c_char_p, C.POINTER(vp), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -170,6 +202,26 @@ class Simulation:
         self.params = self.scene_params[0]
         self.dims = tuple(self.params.dims)
         self.offsets = np.r_[0, np.cumsum(self.scene_n)].astype(np.int64)
+
+    @classmethod
+    def from_snapshot(cls, path):
+        """-> (Simulation, step): a context with the snapshot's scenes and state."""
+        from . import snapshot as _snap
+
+        meta = _snap.read(path, particles=False)
+        self = cls.__new__(cls)
+        self.L = lib()
+        h = C.c_void_p()
+        step = C.c_int64()
+        self._check(self.L.mpm_create_from_snapshot(os.fsencode(path), C.byref(h), C.byref(step)), h)
+        self.ctx = h
+        self.scene_params = [_snap.params_struct(sc["params"]) for sc in meta["scenes"]]
+        self.scene_n = [sc["n"] for sc in meta["scenes"]]
+        self.n = sum(self.scene_n)
+        self.params = self.scene_params[0]
+        self.dims = tuple(self.params.dims)
+        self.offsets = np.r_[0, np.cumsum(self.scene_n)].astype(np.int64)
+        return self, step.value
 
     def _check(self, rc, h=None):
         if rc != MPM_OK:
@@ -252,6 +304,57 @@ class Simulation:
                    FE=np.zeros((n, 9), np.float32), FP=np.zeros((n, 9), np.float32), V0=np.zeros(n, np.float32))
         self._check(self.L.mpm_get_particles(self.ctx, *[_fp(out[k]) for k in ("x", "v", "B", "FE", "FP", "V0")]))
         return out
+
+    # ---- device generators, readback, snapshots
+    def generate(self, kind, seed=0, scene=0, centre=(0.5, 0.5, 0.5), radius=0.1, cell_lo=(0, 0, 0),
+                 cells=(1, 1, 1), ppc=(4, 12), velocity=(0.0, 0.0, 0.0), mass=1e-4, E=(1.4e5, 1.4e5)):
+        """Seeded placement of a scene's particles on the device (GEN_BALL,
+        GEN_CELLS, GEN_LATTICE); -> candidates kept (LATTICE) or n."""
+        g = Generator()
+        g.kind, g.seed = int(kind), int(seed)
+        g.centre[:] = [float(c) for c in centre]
+        g.radius = float(radius)
+        g.cell_lo[:] = [int(c) for c in cell_lo]
+        g.cells[:] = [int(c) for c in cells]
+        g.ppc_lo, g.ppc_hi = int(ppc[0]), int(ppc[1])
+        g.velocity[:] = [float(c) for c in velocity]
+        g.mass = float(mass)
+        g.E_lo, g.E_hi = float(E[0]), float(E[1])
+        kept = C.c_int64()
+        self._check(self.L.mpm_scene_generate(self.ctx, int(scene), C.byref(g), C.byref(kept)))
+        return kept.value
+
+    def scene_material(self, scene=0):
+        """-> (m, E0) of one scene in caller order."""
+        n = self.scene_n[scene]
+        m, E0 = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        self._check(self.L.mpm_scene_get_material(self.ctx, int(scene), _fp(m), _fp(E0)))
+        return m, E0
+
+    def positions(self):
+        """All positions (n, 3) in caller order through the pinned mirror."""
+        x = np.zeros((self.n, 3), np.float32)
+        self._check(self.L.mpm_get_positions(self.ctx, _fp(x)))
+        return x
+
+    def positions_begin(self):
+        """Enqueue a positions readback that overlaps later substeps."""
+        self._check(self.L.mpm_positions_begin(self.ctx))
+
+    def positions_wait(self):
+        """-> (n, 3) view of the pinned mirror (valid until the next begin)."""
+        p = _F()
+        self._check(self.L.mpm_positions_wait(self.ctx, C.byref(p)))
+        return np.ctypeslib.as_array(p, shape=(self.n, 3))
+
+    def save(self, path, step=0):
+        self._check(self.L.mpm_snapshot_save(self.ctx, os.fsencode(path), int(step)))
+
+    def load(self, path):
+        """Replace the state from a snapshot of the same scenes; -> step."""
+        step = C.c_int64()
+        self._check(self.L.mpm_snapshot_load(self.ctx, os.fsencode(path), C.byref(step)))
+        return step.value
 
     # ---- batched scenes
     @property
