@@ -112,14 +112,21 @@ def main():
     import mpm_amd as M
     from mpm_amd import scenes
 
+    t0 = time.perf_counter()
     sc = scenes.slab()
+    host_gen_ms = (time.perf_counter() - t0) * 1e3
     n = sc["x"].shape[0]
     prm = sc["params"]
     p = M.default_params(sc["dims"], sc["h"], compose=M.COMPOSE_LITERAL, bc_mode=prm["bc_mode"], E=prm["E"],
                          nu=prm["nu"], xi=prm["xi"], theta_c=prm["theta_c"], theta_s=prm["theta_s"])
     sim = M.Simulation(p, n)
+    t0 = time.perf_counter()
     sim.set_particles(sc["x"], sc["v"], sc["m"], sc["E0"])
+    upload_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
     sim.init_volumes()
+    sim.sync()
+    volumes_ms = (time.perf_counter() - t0) * 1e3
     V0 = sim.particles()["V0"] if not args.no_cpu_baseline else None
     dt = sc["dt"]
 
@@ -177,11 +184,51 @@ def main():
     traffic = _profiled_traffic(dom)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
+    out["setup_io"] = setup_and_io(M, sim, sc, dt, host_gen_ms, upload_ms, volumes_ms)
     if args.copy_probe:
         sim.copy_probe(1 << 30)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, V0, args.cpu_threads)
     print(json.dumps(out), flush=True)
+
+
+def setup_and_io(M, sim, sc, dt, host_gen_ms, upload_ms, volumes_ms):
+    """Setup and readback around the timed region (not part of `value`):
+    host generation + upload against the device generator on the same-size
+    slab, and the viewer's positions readback (synchronous, and begun before
+    10 substeps so the PCIe copy overlaps them).  Leaves the context holding
+    the device-generated slab."""
+    n = sim.n
+    io = {"host_generate_ms": host_gen_ms, "host_upload_ms": upload_ms, "init_volumes_ms": volumes_ms}
+    sim.positions()  # allocates the staging buffer and pinned mirror
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sim.positions()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    io["positions_readback_ms"] = ms
+    io["positions_readback_GBps"] = 12.0 * n / (ms * 1e6)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.substeps(dt, 10)
+    sim.sync()
+    io["substeps10_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    sim.positions_begin()
+    sim.substeps(dt, 10)
+    sim.positions_wait()
+    sim.sync()
+    io["substeps10_with_async_readback_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    sim.generate(M.GEN_CELLS, seed=2, cell_lo=(64, 128, 64), cells=(128, 32, 128), ppc=(4, 12),
+                 mass=float(sc["m"][0]), E=(1e5, 2e5))
+    sim.sync()
+    io["device_generate_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    sim.init_volumes()
+    sim.sync()
+    io["device_init_volumes_ms"] = (time.perf_counter() - t0) * 1e3
+    return io
 
 
 def bench_batch(args):

@@ -11,10 +11,10 @@
 //   G2P   v' = sum_i w v_i,  B' = sum_i w v_i (X_i - x)^T
 //   x'    = clamp(x + dt v')
 // The reverse runs these backwards from the stored state of substep n:
-//   k_adj_grid_fwd  grid velocities of substep n from the re-run P2G sums
 //   k_adj_g2p_block advection and G2P (mpm_fused.hip, on the block halos):
-//                   node velocity adjoints and the position adjoint through
-//                   the weights
+//                   node velocities from the substep's saved P2G sums, node
+//                   velocity adjoints and the position adjoint through the
+//                   weights
 //   k_adj_grid      wall mask and v = P/m + dt g: adjoints of (m_i, P_i)
 //   k_adj_p2g       P2G (gathered), stress, hardening and the F update:
 //                   adjoints of x, v, B, F_E, F_P of substep n and of E0
@@ -110,28 +110,6 @@ __device__ __forceinline__ bool svd_of(const M3& F, M3& Lb, float (&s)[3], M3& R
 }
 
 // ------------------------------------------------------------------ kernels
-
-// grid velocities of the re-run substep from its P2G sums (gacc kept)
-__global__ void __launch_bounds__(TPB) k_adj_grid_fwd(KArgs a) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.nnodes) return;
-  const float4 acc = a.gacc[g];
-  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (acc.x != 0.0f) {
-    const Scene S = a.scenes[a.blk_scene[g >> 6]];
-    F3 v = f3(acc.y / acc.x + a.dt * S.g[0], acc.z / acc.x + a.dt * S.g[1], acc.w / acc.x + a.dt * S.g[2]);
-    if (S.bc_mode & MPM_BC_DOMAIN) {
-      const int64_t lb = (g >> 6) - S.blk0;
-      const int l = (int)(g & 63);
-      const int bk = (int)(lb % S.nb[2]);
-      const int64_t t2 = lb / S.nb[2];
-      const int bj = (int)(t2 % S.nb[1]), bi = (int)(t2 / S.nb[1]);
-      v = domain_walls(bi * 4 + (l >> 4), bj * 4 + ((l >> 2) & 3), bk * 4 + (l & 3), S.dims[0], S.dims[1], S.dims[2], v);
-    }
-    out = make_float4(v.x, v.y, v.z, acc.x);
-  }
-  a.gvel[g] = out;
-}
 
 // wall mask and v = P/m + dt g: node adjoints (dL/dm, dL/dP) in place
 __global__ void __launch_bounds__(TPB) k_adj_grid(KArgs a, float4* gnode) {
@@ -331,7 +309,6 @@ __global__ void __launch_bounds__(TPB) k_rows_to_aos(float* dst, const float* sr
 
 }  // namespace
 
-void launch_adj_grid_fwd(const KArgs& a, hipStream_t s) { k_adj_grid_fwd<<<nblk(a.nnodes), TPB, 0, s>>>(a); }
 void launch_adj_grid(const KArgs& a, float4* gnode, hipStream_t s) {
   k_adj_grid<<<nblk(a.nnodes), TPB, 0, s>>>(a, gnode);
 }

@@ -8,7 +8,6 @@ namespace mpm {
 // F_P), same layout and stride as the particle buffer.
 enum AdjField : int { AX = FX, AV = FV, AB = FB, AFE = FFE, AFP = FFP, NA = FM };
 
-void launch_adj_grid_fwd(const KArgs& a, hipStream_t s);
 void launch_adj_grid(const KArgs& a, float4* gnode, hipStream_t s);
 void launch_adj_p2g(const KArgs& a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE,
                     hipStream_t s);

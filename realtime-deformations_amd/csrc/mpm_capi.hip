@@ -1053,7 +1053,7 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   const int every = ctx->adj_every, steps = ctx->adj_steps;
   hipStream_t s = ctx->stream;
   float *Gc = nullptr, *Gs = nullptr, *Gs2 = nullptr, *gEs = nullptr, *gEc = nullptr, *seg = nullptr, *tmp = nullptr;
-  float4* gnode = nullptr;
+  float4 *gnode = nullptr, *gsave = nullptr;
   const size_t rows = sizeof(float) * NA * sd;
   bool ok = true;
   auto alloc = [&](auto** ptr, size_t bytes) {
@@ -1064,12 +1064,15 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   alloc(&Gs2, rows);
   alloc(&gEs, sizeof(float) * sd);
   alloc(&gEc, sizeof(float) * sd);
-  alloc(&seg, rows * (size_t)every);
+  // states of a segment's substeps plus the positions after it; the P2G
+  // sums of each substep
+  alloc(&seg, rows * (size_t)(every + 1));
   alloc(&tmp, sizeof(float) * 3 * (size_t)std::max(1, n));
   alloc(&gnode, sizeof(float4) * (size_t)ctx->nnodes);
+  alloc(&gsave, sizeof(float4) * (size_t)ctx->nnodes * every);
   int rc = MPM_OK;
   auto cleanup = [&] {
-    void* b[] = {Gc, Gs, Gs2, gEs, gEc, seg, tmp, gnode};
+    void* b[] = {Gc, Gs, Gs2, gEs, gEc, seg, tmp, gnode, gsave};
     for (void* p : b) (void)hipFree(p);
   };
   if (!ok) {
@@ -1094,32 +1097,35 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   for (int c = nck - 1; c >= 0 && !rc; --c) {
     const int ns = std::min(every, steps - c * every);
     if ((rc = restore_rows(ctx, ctx->adj_ckpt[c])) || (rc = rebin(ctx))) break;
-    // states of the segment's substeps, in its slot order
+    if (!ctx->gacc_clean) reset_grid(ctx);
+    // re-run the segment one substep at a time (P2G pass, grid pass, G2P
+    // pass: mpm_substeps(dt, 1)), keeping each substep's starting state and
+    // its P2G sums, in the segment's slot order
     for (int k = 0; k < ns && !rc; ++k) {
       MPM_HIP(hipMemcpyAsync(seg + (size_t)k * NA * sd, ctx->P, rows, hipMemcpyDeviceToDevice, s));
-      rc = mpm_substeps(ctx, dt, 1);
+      const BlockArgs ba = block_args(ctx, dt);
+      launch_block_pass(ba, PASS_P2G, s);
+      MPM_HIP(hipMemcpyAsync(gsave + (size_t)k * ctx->nnodes, ctx->gacc, sizeof(float4) * ctx->nnodes,
+                             hipMemcpyDeviceToDevice, s));
+      launch_grid_active(ba, s);
+      launch_advance_step(ctx->dstep, ctx->gcnt, s);
+      launch_block_pass(ba, PASS_G2P, s);
+      ++ctx->since_sort;
     }
-    if (rc) break;
+    MPM_HIP(hipGetLastError());
+    MPM_HIP(hipMemcpyAsync(seg + (size_t)ns * NA * sd + (int64_t)AX * sd, ctx->P + (int64_t)FX * sd,
+                           sizeof(float) * 3 * sd, hipMemcpyDeviceToDevice, s));
     launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
     MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
     for (int k = ns - 1; k >= 0; --k) {
       const float* St = seg + (size_t)k * NA * sd;
-      // re-run the P2G of this substep on a scratch copy (the pass also
-      // writes the F update back into the buffer it reads)
-      MPM_HIP(hipMemcpyAsync(ctx->P2, St, rows, hipMemcpyDeviceToDevice, s));
-      MPM_HIP(hipMemcpyAsync(ctx->P2 + (int64_t)FM * sd, ctx->P + (int64_t)FM * sd, sizeof(float) * 3 * sd,
-                             hipMemcpyDeviceToDevice, s));
-      launch_clear(ctx->gacc, ctx->nnodes, s);
+      const float* Xn = seg + (size_t)(k + 1) * NA * sd;
       BlockArgs ba = block_args(ctx, dt);
-      ba.a.P = ctx->P2;
-      launch_block_pass(ba, PASS_P2G, s);
-      KArgs a = make_args(ctx, dt);
-      launch_adj_grid_fwd(a, s);
+      ba.a.gacc = gsave + (size_t)k * ctx->nnodes;
       launch_clear(gnode, ctx->nnodes, s);
-      ba.a.P = ctx->P;
-      launch_adj_g2p_block(ba, St, Gs, Gs2, gnode, s);
-      launch_adj_grid(a, gnode, s);
-      launch_adj_p2g(a, St, Gs, Gs2, gnode, gEs, s);
+      launch_adj_g2p_block(ba, St, Xn, Gs, Gs2, gnode, s);
+      launch_adj_grid(ba.a, gnode, s);
+      launch_adj_p2g(ba.a, St, Gs, Gs2, gnode, gEs, s);
       std::swap(Gs, Gs2);
     }
     launch_rows_to_caller(Gc, Gs, ctx->pid, sd, n, NA, s);

@@ -373,13 +373,24 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 1) k_blo
   }
 }
 
+// grid velocity of a node from its P2G sums (m, P), as k_grid_active
+// computes it for a scene without colliders: v = P / m + dt g, domain walls
+__device__ __forceinline__ F3 node_velocity(float4 acc, const Scene& S, float dt, int i, int j, int k) {
+  if (acc.x == 0.0f) return f3(0.f, 0.f, 0.f);
+  F3 v = f3(acc.y / acc.x + dt * S.g[0], acc.z / acc.x + dt * S.g[1], acc.w / acc.x + dt * S.g[2]);
+  if (S.bc_mode & MPM_BC_DOMAIN) v = domain_walls(i, j, k, S.dims[0], S.dims[1], S.dims[2], v);
+  return v;
+}
+
 // Reverse of advection + G2P for the blocks of the current binning (see
-// launch_adj_g2p_block).  The velocities are those of the re-run substep
-// (gvel); the stage order and notation are those of mpm_adjoint.hip.
+// launch_adj_g2p_block).  Node velocities come from the substep's saved P2G
+// sums (a.gacc); the advection clamp was active on an axis exactly when the
+// next stored position Xn sits on the clamp bound.  The stage order and
+// notation are those of mpm_adjoint.hip.
 template <int BATCHED>
 __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float* __restrict__ St,
-                                                       const float* __restrict__ Gin, float* __restrict__ Gout,
-                                                       float4* __restrict__ gnode) {
+                                                       const float* __restrict__ Xn, const float* __restrict__ Gin,
+                                                       float* __restrict__ Gout, float4* __restrict__ gnode) {
   __shared__ float vh[3 * PLANE];
   __shared__ double gh[3 * PLANE];
   const KArgs& a = ba.a;
@@ -399,13 +410,13 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
     for (int q = t; q < HN * HN * HN; q += TPB) {
       const int hx = q / (HN * HN), hy = (q / HN) % HN, hz = q % HN;
       const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
-      float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+      F3 v = f3(0.f, 0.f, 0.f);
       if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2])
-        g = a.gvel[node_index(gi, gj, gk, S)];
+        v = node_velocity(a.gacc[node_index(gi, gj, gk, S)], S, a.dt, gi, gj, gk);
       const int l = hx * SX + hy * SY + hz;
-      vh[l] = g.x;
-      vh[PLANE + l] = g.y;
-      vh[2 * PLANE + l] = g.z;
+      vh[l] = v.x;
+      vh[PLANE + l] = v.y;
+      vh[2 * PLANE + l] = v.z;
     }
     for (int q = t; q < 3 * PLANE; q += TPB) gh[q] = 0.0;
     __syncthreads();
@@ -423,29 +434,11 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
         fits = fits && lb[d] >= 0 && lb[d] + 3 < HN;
       }
       const int q0 = lb[0] * SX + lb[1] * SY + lb[2];
-      auto fetch = [&](int ia, int ib, int ic) {
-        if (fits) {
-          const int q = q0 + ia * SX + ib * SY + ic;
-          return f3(vh[q], vh[PLANE + q], vh[2 * PLANE + q]);
-        }
-        const float4 g = a.gvel[node_index(ax[0].base - 1 + ia, ax[1].base - 1 + ib, ax[2].base - 1 + ic, S)];
-        return f3(g.x, g.y, g.z);
-      };
-      // forward G2P velocity, for the advection clamp
-      F3 vn = f3(0.f, 0.f, 0.f);
-#pragma unroll
-      for (int ia = 0; ia < 4; ++ia)
-#pragma unroll
-        for (int ib = 0; ib < 4; ++ib)
-#pragma unroll
-          for (int ic = 0; ic < 4; ++ic) vn = vn + fetch(ia, ib, ic) * (ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic]);
-      // refetch the node velocities below instead of keeping all 64 live
-      __asm__ __volatile__("" ::: "memory");
-      const float xs[3] = {x.x, x.y, x.z}, vs[3] = {vn.x, vn.y, vn.z};
       float gx[3], gvv[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const float xn = xs[k] + vs[k] * a.dt;
+        // x' = clamp(x + dt v', 3h, (dims - 3) h): the bound when clamped
+        const float xn = Xn[(int64_t)(FX + k) * sd + p];
         const float lo = 3.0f * S.h, hi = (float)(S.dims[k] - 3) * S.h;
         const float mask = (xn > lo && xn < hi) ? 1.0f : 0.0f;
         const float gxn = Gin[(int64_t)(FX + k) * sd + p];
@@ -493,9 +486,9 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
         for (int ia = 0; ia < 4; ++ia)
           for (int ib = 0; ib < 4; ++ib)
             for (int ic = 0; ic < 4; ++ic) {
-              const int64_t g = node_index(ax[0].base - 1 + ia, ax[1].base - 1 + ib, ax[2].base - 1 + ic, S);
-              const float4 gv4 = a.gvel[g];
-              const F3 wu = node(ia, ib, ic, f3(gv4.x, gv4.y, gv4.z));
+              const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib, k = ax[2].base - 1 + ic;
+              const int64_t g = node_index(i, j, k, S);
+              const F3 wu = node(ia, ib, ic, node_velocity(a.gacc[g], S, a.dt, i, j, k));
               float* gn = reinterpret_cast<float*>(gnode + g);
               atomicAdd(gn + 1, wu.x);
               atomicAdd(gn + 2, wu.y);
@@ -607,12 +600,12 @@ void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s) {
     launch_block_pass_t<1>(ba, mode, s);
 }
 
-void launch_adj_g2p_block(const BlockArgs& ba, const float* St, const float* Gin, float* Gout, float4* gnode,
-                          hipStream_t s) {
+void launch_adj_g2p_block(const BlockArgs& ba, const float* St, const float* Xn, const float* Gin, float* Gout,
+                          float4* gnode, hipStream_t s) {
   if (ba.a.nscenes == 1)
-    k_adj_g2p_block<0><<<ba.grid, TPB, 0, s>>>(ba, St, Gin, Gout, gnode);
+    k_adj_g2p_block<0><<<ba.grid, TPB, 0, s>>>(ba, St, Xn, Gin, Gout, gnode);
   else
-    k_adj_g2p_block<1><<<ba.grid, TPB, 0, s>>>(ba, St, Gin, Gout, gnode);
+    k_adj_g2p_block<1><<<ba.grid, TPB, 0, s>>>(ba, St, Xn, Gin, Gout, gnode);
 }
 
 void launch_grid_active(const BlockArgs& ba, hipStream_t s) {

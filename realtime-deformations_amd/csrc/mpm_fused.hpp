@@ -23,13 +23,14 @@ struct BlockArgs {
 enum { PASS_P2G = 0, PASS_FUSED = 1, PASS_G2P = 2, PASS_MASS = 3 };
 
 // Reverse mode's advection + G2P adjoint on the block structure: per block,
-// node velocities from an LDS halo and the node-velocity adjoints (w (gv' +
-// gB' d)) accumulated in an FP64 LDS halo, flushed with one set of global
-// atomics per block.  St: stored state of the substep (slot order); Gin: the
-// adjoints of the next state; Gout.x receives the partial position adjoint;
-// gnode.yzw the node adjoints.
-void launch_adj_g2p_block(const BlockArgs& ba, const float* St, const float* Gin, float* Gout, float4* gnode,
-                          hipStream_t s);
+// node velocities (from the substep's P2G sums in ba.a.gacc) in an LDS halo
+// and the node-velocity adjoints (w (gv' + gB' d)) accumulated in an FP64 LDS
+// halo, flushed with one set of global atomics per block.  St: stored state
+// of the substep (slot order); Xn: rows holding the next state's positions;
+// Gin: the adjoints of the next state; Gout.x receives the partial position
+// adjoint; gnode.yzw the node adjoints.
+void launch_adj_g2p_block(const BlockArgs& ba, const float* St, const float* Xn, const float* Gin, float* Gout,
+                          float4* gnode, hipStream_t s);
 void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s);
 void launch_grid_active(const BlockArgs& ba, hipStream_t s);
 void launch_advance_step(int* dstep, int* gcnt, hipStream_t s);
