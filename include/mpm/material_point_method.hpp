@@ -277,7 +277,8 @@ struct LagrangeEulerView : Loggable {
   void commitParticles();
   // n fused substeps at once (the canonical seven calls n times).
   void substeps(ftype timeDelta, int n, const std::vector<MeshCollider>& objects);
-  // Positions only, x y z per particle (what a viewer reads).
+  // Positions only, x y z per particle (what a viewer reads), from the
+  // context's pinned host mirror: valid until the next getPositions().
   const float* getPositions();
   mpm_ctx* context() { return ctx; }
   // Seed of the placement generator used by initializeParticles.

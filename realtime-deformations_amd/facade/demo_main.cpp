@@ -129,6 +129,9 @@ int main(int argc, char** argv) {
     cz += p[i].pos.z;
   }
   dump(prefix + ".final.f32", fin);
+  // the viewer's positions-only read (pinned mirror)
+  const float* pos = MPM.getPositions();
+  dump(prefix + ".positions.f32", std::vector<float>(pos, pos + 3 * (size_t)numParticles));
   std::printf("frames %d  ms/frame %.4f  centre %.6f %.6f %.6f\n", frames, ms / frames, cx / numParticles,
               cy / numParticles, cz / numParticles);
   return 0;

@@ -303,9 +303,17 @@ Particle* LagrangeEulerView::getParticles() {
 
 const float* LagrangeEulerView::getPositions() {
   flush();
-  positions.resize(3 * (size_t)nParticles);
-  check(mpm_get_particles(ctx, positions.data(), nullptr, nullptr, nullptr, nullptr, nullptr), "getPositions");
-  return positions.data();
+  // gather + copy into the context's pinned mirror; no host-side copy
+  const float* view = nullptr;
+  int rc = mpm_positions_begin(ctx);
+  if (rc == MPM_OK) rc = mpm_positions_wait(ctx, &view);
+  if (rc == MPM_OK) rc = mpm_sync(ctx);
+  check(rc, "getPositions");
+  if (rc != MPM_OK || !view) {
+    positions.assign(3 * (size_t)nParticles, 0.0f);
+    return positions.data();
+  }
+  return view;
 }
 
 void LagrangeEulerView::precalculateWeights() {

@@ -34,6 +34,8 @@ def test_facade_demo_loop_matches_oracle(tmp_path):
     s.stage(O.STAGE_VOLUMES)
     s.substeps(1e-5, frames)
     ref = s.particles()
+    pos = np.fromfile(prefix + ".positions.f32", np.float32).reshape(-1, 3)
+    np.testing.assert_array_equal(pos, fin[:, 0:3])  # getPositions() == getParticles() positions
     assert rel_err(fin[:, 0:3], ref["x"]) < 1e-5
     assert rel_err(fin[:, 3:6], ref["v"]) < 1e-4
     assert rel_err(fin[:, 6:15], ref["FE"]) < 1e-4
