@@ -227,24 +227,47 @@ __device__ __forceinline__ int select_bit(unsigned long long m, int k) {
   return pos;
 }
 
-// Move every listed block's particles to their round-robin slots: slot
-// start + R[r] + k holds the r-th particle of the k-th cell that has more than
-// r particles, R[r] = sum_c' min(cnt[c'], r) being the particles of earlier
-// rounds.  One wave tabulates, per round, the ballot of cells that still have
-// particles and R; every thread then walks destination slots (coalesced
-// writes) and finds its source with one bit select.  Blocks with a cell of
-// more than RR_MAX particles take the per-source formula instead.
+// Slot layout inside a block, chosen for the LDS banks of the block pass.
+// A particle's stencil origin in the halo sets the FP64 bank pair of all its
+// atomics by its cell's column (cy, cz); a wave's 64 lanes issue an atomic as
+// four 16-lane groups, each free of conflicts when its 16 lanes come from 16
+// different columns.  The slots are therefore cut into groups of 16, and
+// each group takes one particle from every column that still has particles
+// (the column's list: depth-major over its four cells cx); when fewer than 16
+// columns remain, the group's spare slots go one by one to the column with
+// the most particles left (lowest column on ties).  Inside a group the
+// columns come in order.  A column's k-th taken particle is entry k of its
+// list; list entry p is the particle of depth d (the d-th of its cell in
+// sorted order) in the cell cx given by the depth's bit mask.
+//
+// Tables (one wave builds them): per column and depth the mask of cells with
+// more than d particles and the list position where depth d starts; per
+// group and column the first slot in the group and the list position at the
+// group's start.  Threads then walk destination slots (coalesced writes) and
+// find their sources by table lookups.  Blocks with a cell of more than
+// RR_MAX particles or more than GMAX groups take the plain round-robin of the
+// per-source formula instead (slot of the r-th particle of cell c:
+// sum_c' min(cnt[c'], r) + #{c' < c : cnt[c'] > r}).
 constexpr int RR_MAX = 64;
+constexpr int GMAX = 256;
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
 __global__ void __launch_bounds__(256) k_reorder(BinArgs b) {
   __shared__ int cnt[64];
   __shared__ int cstart[64];
-  __shared__ unsigned long long rmask[RR_MAX];
-  __shared__ int rbase[RR_MAX + 1];
-  __shared__ int nround;
+  __shared__ unsigned char dmask[16][RR_MAX];
+  __shared__ unsigned short dbase[16][RR_MAX + 1];
+  __shared__ unsigned char gofs[GMAX][16];
+  __shared__ unsigned short gptr[GMAX][16];
+  __shared__ int maxc_s, table_s;
   const int t = threadIdx.x;
   for (int s = blockIdx.x; s < *b.nblk; s += gridDim.x) {
     const int blk = b.blk_ids[s];
     const int start = b.blk_start[blk], end = b.blk_end[blk];
+    const int nb = end - start;
     if (t < 64) cnt[t] = 0;
     __syncthreads();
     for (int i = start + t; i < end; i += 256) atomicAdd(&cnt[b.keys[i] & 63u], 1);
@@ -257,31 +280,70 @@ __global__ void __launch_bounds__(256) k_reorder(BinArgs b) {
         if (t >= off) incl += v;
       }
       cstart[t] = incl - c;
-      int base = 0, r = 0;
-      for (; r < RR_MAX; ++r) {
-        const unsigned long long m = __ballot(c > r);
-        if (m == 0ull) break;
-        if (t == 0) {
-          rmask[r] = m;
-          rbase[r] = base;
-        }
-        base += __popcll(m);
-      }
-      const bool more = __ballot(c > RR_MAX) != 0ull;
+      const int maxc = wave_max(c);
+      const bool table = maxc <= RR_MAX && (nb + 15) / 16 <= GMAX;
       if (t == 0) {
-        rbase[r] = base;
-        nround = more ? -1 : r;
+        maxc_s = maxc;
+        table_s = table ? 1 : 0;
+      }
+      if (table) {
+        // column lists (lanes 0..15: column g)
+        int ncol = 0;
+        if (t < 16) {
+          for (int d = 0; d < maxc; ++d) {
+            int m = 0;
+#pragma unroll
+            for (int cx = 0; cx < 4; ++cx) m |= (cnt[(cx << 4) | t] > d ? 1 : 0) << cx;
+            dmask[t][d] = (unsigned char)m;
+            dbase[t][d] = (unsigned short)ncol;
+            ncol += __popc(m);
+          }
+          dbase[t][maxc] = (unsigned short)ncol;
+        }
+        // groups of 16 slots (the whole wave runs the reductions; lanes >= 16 hold nothing)
+        int rem = t < 16 ? ncol : 0, ptr = 0, left = nb;
+        for (int gi = 0; left > 0; ++gi) {
+          const int size = left < 16 ? left : 16;
+          int take = rem > 0 ? 1 : 0;
+          const int k = __popcll(__ballot(take != 0));
+          for (int e = size - k; e > 0; --e) {
+            const int key = (t < 16 && rem - take > 0) ? (((rem - take) << 6) | (63 - t)) : -1;
+            const int best = wave_max(key);
+            if (key >= 0 && key == best) ++take;
+          }
+          int pre = take;
+#pragma unroll
+          for (int off = 1; off < 16; off <<= 1) {
+            const int v = __shfl_up(pre, off, 64);
+            if ((t & 15) >= off) pre += v;
+          }
+          if (t < 16) {
+            gofs[gi][t] = (unsigned char)(pre - take);
+            gptr[gi][t] = (unsigned short)ptr;
+          }
+          ptr += take;
+          rem -= take;
+          left -= size;
+        }
       }
     }
     __syncthreads();
-    const int nr = nround;
-    if (nr >= 0) {
+    if (table_s) {
+      const int maxc = maxc_s;
       for (int dst = start + t; dst < end; dst += 256) {
-        const int j = dst - start;
-        int r = 0;
-        while (r + 1 < nr && rbase[r + 1] <= j) ++r;
-        const int c = select_bit(rmask[r], j - rbase[r]);
-        const int src = (int)b.vals[start + cstart[c] + r];
+        const int j = dst - start, gi = j >> 4, r = j & 15;
+        int g = 15;
+        while (gofs[gi][g] > r) --g;
+        const int p = gptr[gi][g] + (r - gofs[gi][g]);
+        int lo = 0, hi = maxc - 1;  // last depth whose list start is <= p
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (dbase[g][mid] <= p) lo = mid;
+          else hi = mid - 1;
+        }
+        const int cx = select_bit((unsigned long long)dmask[g][lo], p - dbase[g][lo]);
+        const int c = (cx << 4) | g;
+        const int src = (int)b.vals[start + cstart[c] + lo];
         for (int f = 0; f < NFIELDS; ++f) b.Pout[(int64_t)f * b.stride + dst] = b.P[(int64_t)f * b.stride + src];
         b.pid_out[dst] = b.pid[src];
       }

@@ -137,16 +137,26 @@ def test_warm_state_one_substep():
     _compare_particles(g.particles(), o.particles(), 1e-5, h=sc["h"])
 
 
-@pytest.mark.parametrize("which", ["reference", "snowball", "two_snowballs", "random"])
+@pytest.mark.parametrize("which", ["reference", "snowball", "two_snowballs", "random", "ragged_slab", "clumped"])
 def test_binning_matches_cpu_stable_sort(which):
     """Keys and permutation of the radix sort equal numpy's stable argsort;
-    the round-robin placement equals its CPU definition."""
+    the in-block slot layout (column groups, and the plain round-robin for a
+    block with a cell of more than 64 particles) equals its CPU definition."""
     if which == "reference":
         sc = scenes.reference_scene()
     elif which == "snowball":
         sc = scenes.snowball()
     elif which == "two_snowballs":
         sc = scenes.two_snowballs()
+    elif which == "ragged_slab":
+        sc = scenes.slab(n_total=2 ** 16, cells=(32, 8, 32), grid=128)
+    elif which == "clumped":
+        rng = np.random.default_rng(12)
+        n = 40000
+        x = rng.uniform(0.2, 0.8, (n, 3)).astype(np.float32)
+        x[:300] = (np.float32([20.5, 21.5, 22.5]) + rng.uniform(-0.45, 0.45, (300, 3)).astype(np.float32)) / 64
+        sc = dict(x=x, v=np.zeros_like(x), m=np.ones(n, np.float32), E0=np.full(n, 1.4e5, np.float32),
+                  dims=(64, 64, 64), h=1 / 64, params=scenes.material())
     else:
         rng = np.random.default_rng(11)
         n = 100003
