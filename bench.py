@@ -355,7 +355,21 @@ def bench_adjoint(args):
     gx0, gv0, gE0 = b.adjoint_backward(gx, gv)
     b.sync()
     bwd_s = time.perf_counter() - t0
+    # the same run keeping every substep's record in HBM (checkpoint_every = 1:
+    # no segment re-run in the backward)
+    b.sync()
+    t0 = time.perf_counter()
+    b.adjoint_forward(dt, steps, 1)
+    b.sync()
+    fwd1_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gx1, gv1, gE1 = b.adjoint_backward(gx, gv)
+    b.sync()
+    bwd1_s = time.perf_counter() - t0
     b.close()
+
+    def rel(a, r):
+        return float(np.linalg.norm(a.astype(np.float64) - r) / max(np.linalg.norm(r.astype(np.float64)), 1e-30))
     out = {
         "metric": "fwd+bwd substeps/s, 16 x 32,768 particles on 64^3 (checkpoints every 20)",
         "value": steps / (fwd_s + bwd_s),
@@ -374,8 +388,16 @@ def bench_adjoint(args):
         "forward_ms_per_substep": fwd_s / steps * 1e3,
         "backward_ms_per_substep": bwd_s / steps * 1e3,
         "plain_forward_ms_per_substep": plain_s / steps * 1e3,
-        "fwd_plus_bwd_over_fwd": (fwd_s + bwd_s) / fwd_s,
+        "fwd_plus_bwd_over_checkpointed_fwd": (fwd_s + bwd_s) / fwd_s,
+        "fwd_plus_bwd_over_plain_fwd": (fwd_s + bwd_s) / plain_s,
         "particle_substeps_per_s_fwd_bwd": n * steps / (fwd_s + bwd_s),
+        "keep_every_substep": {
+            "forward_ms_per_substep": fwd1_s / steps * 1e3,
+            "backward_ms_per_substep": bwd1_s / steps * 1e3,
+            "fwd_plus_bwd_over_plain_fwd": (fwd1_s + bwd1_s) / plain_s,
+            "record_bytes_per_substep": 132 * n + 16 * sum(int(np.prod(sc["dims"])) for sc in scs),
+            "grad_rel_diff_vs_checkpoints": {"x0": rel(gx1, gx0), "v0": rel(gv1, gv0), "E0": rel(gE1, gE0)},
+        },
         "grad_finite": bool(np.isfinite(gx0).all() and np.isfinite(gv0).all() and np.isfinite(gE0).all()),
     }
     print(json.dumps(out), flush=True)

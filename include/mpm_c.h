@@ -237,7 +237,10 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms_per_subs
 /* ---- reverse mode (no reference counterpart: the reference has no
  * gradients).  mpm_adjoint_forward runs nsteps substeps from the current
  * state, keeping a checkpoint every `checkpoint_every` substeps (device
- * memory: 132 bytes per particle per checkpoint).  mpm_adjoint_backward takes
+ * memory: 132 bytes per particle per checkpoint; checkpoint_every = 1
+ * instead keeps every substep's record, 132 bytes per particle plus 16 bytes
+ * per grid node per substep, and the backward re-runs nothing).
+ * mpm_adjoint_backward takes
  * dL/dx and dL/dv of the final particles (n x 3, caller order; NULL = 0) and
  * returns dL/dx0, dL/dv0 (n x 3) and dL/dE0 (n) of the state the forward
  * started from (NULL skips), recomputing each segment from its checkpoint; it

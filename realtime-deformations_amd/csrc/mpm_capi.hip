@@ -431,6 +431,10 @@ int mpm_destroy(mpm_ctx* ctx) {
   for (DevCollider* c : ctx->scoll) (void)hipFree(c);
   for (float* c : ctx->adj_ckpt) (void)hipFree(c);
   (void)hipFree(ctx->adj_const);
+  for (auto& g : ctx->adj_segs) {
+    void* b[] = {g.st, g.gs, g.pid, g.blist, g.nblk};
+    for (void* q : b) (void)hipFree(q);
+  }
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->ev_region)
     if (e) (void)hipEventDestroy(e);
@@ -595,6 +599,8 @@ BlockArgs block_args(mpm_ctx* ctx, float dt) {
   ba.dstep = ctx->dstep;
   ba.nblocks = ctx->nblocks;
   ba.grid = ctx->block_grid;
+  ba.rec = nullptr;
+  ba.keep_acc = 0;
   return ba;
 }
 
@@ -1013,6 +1019,61 @@ int restore_rows(mpm_ctx* ctx, const float* rows33) {
   return MPM_OK;
 }
 
+// Run ns substeps from the current binning on the fused path (the first one
+// a P2G pass, then fused passes, then a final G2P pass: mpm_substeps(dt, ns)
+// without re-binning), the passes writing each substep's starting state
+// (rows 0..32 in slot order, seg + k * 33 rows) and the positions after the
+// last one (seg + ns * 33 rows), the P2G sums going straight into
+// gsave + k * nnodes and staying there.
+int record_segment(mpm_ctx* ctx, float dt, int ns, float* seg, float4* gsave) {
+  const int64_t sd = ctx->stride;
+  hipStream_t s = ctx->stream;
+  if (!ctx->gacc_clean) reset_grid(ctx);
+  MPM_HIP(hipMemsetAsync(gsave, 0, sizeof(float4) * (size_t)ctx->nnodes * ns, s));
+  for (int k = 0; k < ns; ++k) {
+    BlockArgs ba = block_args(ctx, dt);
+    ba.a.gacc = gsave + (size_t)k * ctx->nnodes;
+    ba.rec = seg + (size_t)k * NA * sd;
+    ba.keep_acc = 1;
+    launch_block_pass(ba, k == 0 ? PASS_P2G : PASS_FUSED, s);
+    launch_grid_active(ba, s);
+    launch_advance_step(ctx->dstep, ctx->gcnt, s);
+    ++ctx->since_sort;
+  }
+  BlockArgs ba = block_args(ctx, dt);
+  ba.rec = seg + (size_t)ns * NA * sd;
+  launch_block_pass(ba, PASS_G2P, s);
+  MPM_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
+void free_segments(mpm_ctx* ctx) {
+  for (auto& g : ctx->adj_segs) {
+    void* b[] = {g.st, g.gs, g.pid, g.blist, g.nblk};
+    for (void* q : b) (void)hipFree(q);
+  }
+  ctx->adj_segs.clear();
+}
+
+// Adjoint kernels over the ns recorded substeps of one segment, last to
+// first; Gs/Gs2 (slot order) swap per substep, Gs holds the result.
+void adjoint_segment(mpm_ctx* ctx, float dt, int ns, const float* seg, float4* gsave, float*& Gs, float*& Gs2,
+                     float4* gnode, float* gEs) {
+  const int64_t sd = ctx->stride;
+  hipStream_t s = ctx->stream;
+  for (int k = ns - 1; k >= 0; --k) {
+    const float* St = seg + (size_t)k * NA * sd;
+    const float* Xn = seg + (size_t)(k + 1) * NA * sd;
+    BlockArgs ba = block_args(ctx, dt);
+    ba.a.gacc = gsave + (size_t)k * ctx->nnodes;
+    launch_clear(gnode, ctx->nnodes, s);
+    launch_adj_g2p_block(ba, St, Xn, Gs, Gs2, gnode, s);
+    launch_adj_grid(ba.a, gnode, s);
+    launch_adj_p2g(ba.a, St, Gs, Gs2, gnode, gEs, s);
+    std::swap(Gs, Gs2);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1023,16 +1084,45 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
   const int64_t sd = ctx->stride;
   for (float* c : ctx->adj_ckpt) (void)hipFree(c);
   ctx->adj_ckpt.clear();
+  free_segments(ctx);
   ctx->adj_steps = 0;
   if (!ctx->adj_const) MPM_HIP(hipMalloc(&ctx->adj_const, sizeof(float) * 3 * sd));
   launch_rows_to_caller(ctx->adj_const, ctx->P + (int64_t)FM * sd, ctx->pid, sd, ctx->n, 3, ctx->stream);
-  const int nck = (nsteps + every - 1) / every;
+  const int nck = every == 1 ? 1 : (nsteps + every - 1) / every;
   for (int c = 0; c < nck; ++c) {
     float* buf = nullptr;
     MPM_HIP(hipMalloc(&buf, sizeof(float) * NA * sd));
     ctx->adj_ckpt.push_back(buf);
     launch_rows_to_caller(buf, ctx->P, ctx->pid, sd, ctx->n, NA, ctx->stream);
-    if (int rc = mpm_substeps(ctx, dt, std::min(every, nsteps - c * every))) return rc;
+    if (every > 1)
+      if (int rc = mpm_substeps(ctx, dt, std::min(every, nsteps - c * every))) return rc;
+  }
+  if (every == 1) {
+    // keep every substep: segments of resort_every substeps, one binning each
+    const int L = std::max(1, ctx->resort_every);
+    const size_t rows = sizeof(float) * NA * sd;
+    for (int done = 0; done < nsteps;) {
+      mpm_ctx::AdjSegment g;
+      g.ns = std::min(L, nsteps - done);
+      bool ok = hipMalloc(&g.st, rows * (size_t)(g.ns + 1)) == hipSuccess &&
+                hipMalloc(&g.gs, sizeof(float4) * (size_t)ctx->nnodes * g.ns) == hipSuccess &&
+                hipMalloc(&g.pid, sizeof(int) * (size_t)std::max(1, ctx->n)) == hipSuccess &&
+                hipMalloc(&g.blist, sizeof(int4) * (size_t)ctx->nblocks) == hipSuccess &&
+                hipMalloc(&g.nblk, sizeof(int)) == hipSuccess;
+      ctx->adj_segs.push_back(g);
+      if (!ok) {
+        free_segments(ctx);
+        ctx->err = "device memory for the per-substep records (checkpoint_every = 1) is exhausted";
+        return MPM_ERR_NOMEM;
+      }
+      if (int rc = rebin(ctx)) return rc;
+      MPM_HIP(hipMemcpyAsync(g.pid, ctx->pid, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
+      MPM_HIP(hipMemcpyAsync(g.blist, ctx->blist, sizeof(int4) * ctx->nblocks, hipMemcpyDeviceToDevice, ctx->stream));
+      MPM_HIP(hipMemcpyAsync(g.nblk, ctx->nblk, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+      if (int rc = record_segment(ctx, dt, g.ns, g.st, g.gs)) return rc;
+      done += g.ns;
+    }
+    ctx->gacc_clean = false;
   }
   ctx->adj_dt = dt;
   ctx->adj_steps = nsteps;
@@ -1066,10 +1156,12 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   alloc(&gEc, sizeof(float) * sd);
   // states of a segment's substeps plus the positions after it; the P2G
   // sums of each substep
-  alloc(&seg, rows * (size_t)(every + 1));
+  if (every > 1) {
+    alloc(&seg, rows * (size_t)(every + 1));
+    alloc(&gsave, sizeof(float4) * (size_t)ctx->nnodes * every);
+  }
   alloc(&tmp, sizeof(float) * 3 * (size_t)std::max(1, n));
   alloc(&gnode, sizeof(float4) * (size_t)ctx->nnodes);
-  alloc(&gsave, sizeof(float4) * (size_t)ctx->nnodes * every);
   int rc = MPM_OK;
   auto cleanup = [&] {
     void* b[] = {Gc, Gs, Gs2, gEs, gEc, seg, tmp, gnode, gsave};
@@ -1093,41 +1185,31 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   }
   const int keep_resort = ctx->resort_every;
   ctx->resort_every = 1 << 30;  // one binning per segment: slots stay fixed inside it
-  const int nck = (steps + every - 1) / every;
+  if (every == 1) {
+    // recorded segments: restore each one's slot order and block list
+    for (int c = (int)ctx->adj_segs.size() - 1; c >= 0 && !rc; --c) {
+      mpm_ctx::AdjSegment& g = ctx->adj_segs[c];
+      MPM_HIP(hipMemcpyAsync(ctx->pid, g.pid, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+      MPM_HIP(hipMemcpyAsync(ctx->blist, g.blist, sizeof(int4) * ctx->nblocks, hipMemcpyDeviceToDevice, s));
+      MPM_HIP(hipMemcpyAsync(ctx->nblk, g.nblk, sizeof(int), hipMemcpyDeviceToDevice, s));
+      // m, V0, E0 in this segment's slot order (k_adj_p2g reads them from P)
+      launch_rows_to_slots(ctx->P + (int64_t)FM * sd, ctx->adj_const, ctx->pid, sd, n, 3, s);
+      launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
+      MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
+      adjoint_segment(ctx, dt, g.ns, g.st, g.gs, Gs, Gs2, gnode, gEs);
+      launch_rows_to_caller(Gc, Gs, ctx->pid, sd, n, NA, s);
+      launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
+    }
+  }
+  const int nck = every == 1 ? 0 : (steps + every - 1) / every;
   for (int c = nck - 1; c >= 0 && !rc; --c) {
     const int ns = std::min(every, steps - c * every);
     if ((rc = restore_rows(ctx, ctx->adj_ckpt[c])) || (rc = rebin(ctx))) break;
-    if (!ctx->gacc_clean) reset_grid(ctx);
-    // re-run the segment one substep at a time (P2G pass, grid pass, G2P
-    // pass: mpm_substeps(dt, 1)), keeping each substep's starting state and
-    // its P2G sums, in the segment's slot order
-    for (int k = 0; k < ns && !rc; ++k) {
-      MPM_HIP(hipMemcpyAsync(seg + (size_t)k * NA * sd, ctx->P, rows, hipMemcpyDeviceToDevice, s));
-      const BlockArgs ba = block_args(ctx, dt);
-      launch_block_pass(ba, PASS_P2G, s);
-      MPM_HIP(hipMemcpyAsync(gsave + (size_t)k * ctx->nnodes, ctx->gacc, sizeof(float4) * ctx->nnodes,
-                             hipMemcpyDeviceToDevice, s));
-      launch_grid_active(ba, s);
-      launch_advance_step(ctx->dstep, ctx->gcnt, s);
-      launch_block_pass(ba, PASS_G2P, s);
-      ++ctx->since_sort;
-    }
-    MPM_HIP(hipGetLastError());
-    MPM_HIP(hipMemcpyAsync(seg + (size_t)ns * NA * sd + (int64_t)AX * sd, ctx->P + (int64_t)FX * sd,
-                           sizeof(float) * 3 * sd, hipMemcpyDeviceToDevice, s));
+    // re-run the segment, keeping each substep's starting state and P2G sums
+    if ((rc = record_segment(ctx, dt, ns, seg, gsave))) break;
     launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
     MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
-    for (int k = ns - 1; k >= 0; --k) {
-      const float* St = seg + (size_t)k * NA * sd;
-      const float* Xn = seg + (size_t)(k + 1) * NA * sd;
-      BlockArgs ba = block_args(ctx, dt);
-      ba.a.gacc = gsave + (size_t)k * ctx->nnodes;
-      launch_clear(gnode, ctx->nnodes, s);
-      launch_adj_g2p_block(ba, St, Xn, Gs, Gs2, gnode, s);
-      launch_adj_grid(ba.a, gnode, s);
-      launch_adj_p2g(ba.a, St, Gs, Gs2, gnode, gEs, s);
-      std::swap(Gs, Gs2);
-    }
+    adjoint_segment(ctx, dt, ns, seg, gsave, Gs, Gs2, gnode, gEs);
     launch_rows_to_caller(Gc, Gs, ctx->pid, sd, n, NA, s);
     launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
     ctx->gacc_clean = false;

@@ -227,7 +227,7 @@ __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S,
 }
 
 // Stress from F^n, F update F^n -> F^{n+1} with B, then the P2G scatter.
-template <int COMPOSE>
+template <int COMPOSE, int REC>
 __device__ __forceinline__ void p2g_one(const BlockArgs& ba, const Scene& S, int scene, double* __restrict__ ah,
                                         const int o[3], int p, F3 x, F3 v, const M3& B) {
   const KArgs& a = ba.a;
@@ -242,6 +242,24 @@ __device__ __forceinline__ void p2g_one(const BlockArgs& ba, const Scene& S, int
         FE.c[c][r] = ld(a, FFE + 3 * c + r, p);
         FP.c[c][r] = ld(a, FFP + 3 * c + r, p);
       }
+    if (REC) {
+      float* R = ba.rec;
+      const int64_t sd = a.stride;
+      R[(int64_t)(FX + 0) * sd + p] = x.x;
+      R[(int64_t)(FX + 1) * sd + p] = x.y;
+      R[(int64_t)(FX + 2) * sd + p] = x.z;
+      R[(int64_t)(FV + 0) * sd + p] = v.x;
+      R[(int64_t)(FV + 1) * sd + p] = v.y;
+      R[(int64_t)(FV + 2) * sd + p] = v.z;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          R[(int64_t)(FB + 3 * c + r) * sd + p] = B.c[c][r];
+          R[(int64_t)(FFE + 3 * c + r) * sd + p] = FE.c[c][r];
+          R[(int64_t)(FFP + 3 * c + r) * sd + p] = FP.c[c][r];
+        }
+    }
     m = ld(a, FM, p);
     const float V0 = ld(a, FVOL, p);
     const float E = ld(a, FE0, p);
@@ -281,7 +299,7 @@ __device__ __forceinline__ F3 advect(const KArgs& a, const Scene& S, F3 x, F3 v)
 // device table, copied once per block.
 // LITERAL kernels are held to three waves per SIMD (<= 168 registers); the
 // TEXTBOOK ones, with their FP64 re-solve, would spill heavily and keep two.
-template <int MODE, int COMPOSE, int BATCHED>
+template <int MODE, int COMPOSE, int BATCHED, int REC = 0>
 __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 1) k_block_pass(BlockArgs ba) {
   constexpr bool kG2P = MODE == MODE_FUSED || MODE == MODE_G2P;
   constexpr bool kP2G = MODE != MODE_G2P;
@@ -333,6 +351,11 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 1) k_blo
         st(a, FX + 1, p, x.y);
         st(a, FX + 2, p, x.z);
         if (MODE == MODE_G2P) {
+          if (REC) {
+            ba.rec[(int64_t)(FX + 0) * a.stride + p] = x.x;
+            ba.rec[(int64_t)(FX + 1) * a.stride + p] = x.y;
+            ba.rec[(int64_t)(FX + 2) * a.stride + p] = x.z;
+          }
           st(a, FV, p, v.x);
           st(a, FV + 1, p, v.y);
           st(a, FV + 2, p, v.z);
@@ -348,7 +371,7 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 1) k_blo
 #pragma unroll
           for (int r = 0; r < 3; ++r) B.c[c][r] = ld(a, FB + 3 * c + r, p);
       }
-      if (kP2G) p2g_one<COMPOSE>(ba, S, scene, ah, o, p, x, v, B);
+      if (kP2G) p2g_one<COMPOSE, REC>(ba, S, scene, ah, o, p, x, v, B);
     }
     __syncthreads();
     if (kP2G) {
@@ -550,7 +573,7 @@ template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(Bloc
         }
         out = make_float4(v.x, v.y, v.z, acc.x);
       }
-      if (acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f || acc.w != 0.0f)
+      if (!ba.keep_acc && (acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f || acc.w != 0.0f))
         a.gacc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
       a.gvel[g] = out;
     } else {
@@ -575,6 +598,14 @@ void launch_advance_step(int* dstep, int* gcnt, hipStream_t s) { k_advance_step<
 template <int BATCHED> void launch_block_pass_t(const BlockArgs& ba, int mode, hipStream_t s) {
   const int grid = ba.grid;
   const bool lit = ba.a.compose == COMPOSE_LITERAL;
+  if (ba.rec) {  // reverse-mode recording: TEXTBOOK only (checked by the caller)
+    switch (mode) {
+      case MODE_P2G: k_block_pass<MODE_P2G, COMPOSE_TEXTBOOK, BATCHED, 1><<<grid, TPB, 0, s>>>(ba); break;
+      case MODE_FUSED: k_block_pass<MODE_FUSED, COMPOSE_TEXTBOOK, BATCHED, 1><<<grid, TPB, 0, s>>>(ba); break;
+      default: k_block_pass<MODE_G2P, COMPOSE_TEXTBOOK, BATCHED, 1><<<grid, TPB, 0, s>>>(ba); break;
+    }
+    return;
+  }
   switch (mode) {
     case MODE_P2G:
       if (lit)

@@ -18,6 +18,12 @@ struct BlockArgs {
   const int* dstep;   // substep epoch (> 0)
   int64_t nblocks;    // grid blocks over all scenes
   int grid;           // workgroups of the persistent block pass
+  // reverse-mode recording (TEXTBOOK passes only): P2G and fused passes
+  // write each particle's substep-start state (33 rows: x, v, B, F_E, F_P)
+  // to rec, the G2P pass its new positions to rec's x rows; with keep_acc
+  // the grid pass leaves the P2G sums in a.gacc instead of clearing them
+  float* rec;
+  int keep_acc;
 };
 
 enum { PASS_P2G = 0, PASS_FUSED = 1, PASS_G2P = 2, PASS_MASS = 3 };

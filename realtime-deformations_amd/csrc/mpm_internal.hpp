@@ -148,6 +148,18 @@ struct mpm_ctx {
   int adj_steps = 0, adj_every = 0;
   std::vector<float*> adj_ckpt;
   float* adj_const = nullptr;
+  // checkpoint_every == 1: every substep's record kept, segment by segment
+  // (one binning each): starting states (+ the positions after the segment),
+  // P2G sums, and the segment's slot order and block list
+  struct AdjSegment {
+    int ns = 0;
+    float* st = nullptr;     // (ns + 1) x 33 rows
+    float4* gs = nullptr;    // ns x nnodes
+    int* pid = nullptr;      // n
+    int4* blist = nullptr;   // nblocks
+    int* nblk = nullptr;     // 1
+  };
+  std::vector<AdjSegment> adj_segs;
   // positions readback: device staging, pinned host mirror, copy stream
   float* pos_stage = nullptr;
   float* pos_host = nullptr;     // pinned

@@ -115,6 +115,34 @@ def test_gradient_wrt_x0_spot_checks(case):
         assert abs(ad - fd) <= 2e-2 * abs(dyn_fd), (p, k, ad, fd, dyn_fd)
 
 
+def test_keep_every_substep_matches_checkpoints(case):
+    """checkpoint_every = 1 keeps every substep's record (state, P2G sums,
+    slot order) and skips the segment re-runs: the same gradients as
+    checkpoints every 5, and the context is back at its initial state.
+    Different checkpoint intervals bin at different substeps, so the FP32
+    grid sums add in another order; measured across intervals 2, 5, 6, 12:
+    dL/dx0 and dL/dv0 agree to ~4e-7, the small per-particle dL/dE0 values to
+    ~3e-4 and their sum (the E0 direction the finite differences check, to
+    2e-3) to ~2e-4.  Tolerances: 1e-5, 1e-3 and 1e-3."""
+    sc = case["sc"]
+    g = _gpu(sc)
+    g.set_particles(sc["x"], sc["v"], sc["m"], sc["E0"])
+    g.init_volumes()
+    g.adjoint_forward(DT, STEPS, 1)
+    gx0, gv0, gE0 = g.adjoint_backward(case["a"], case["b"])
+    np.testing.assert_array_equal(g.particles()["x"], sc["x"])
+    for got, ref, tol in ((gx0, case["gx0"], 1e-5), (gv0, case["gv0"], 1e-5), (gE0, case["gE0"], 1e-3)):
+        err = np.linalg.norm(got.astype(np.float64) - ref) / np.linalg.norm(ref.astype(np.float64))
+        assert err < tol, (err, tol)
+    dE, dE_ref = float(np.sum(gE0.astype(np.float64))), float(np.sum(case["gE0"].astype(np.float64)))
+    assert abs(dE - dE_ref) <= 1e-3 * abs(dE_ref), (dE, dE_ref)
+    # and a second backward from the same records gives the same result
+    g.adjoint_forward(DT, STEPS, 1)
+    gx1, _, _ = g.adjoint_backward(case["a"], case["b"])
+    assert np.linalg.norm(gx1.astype(np.float64) - gx0) <= 1e-5 * np.linalg.norm(gx0.astype(np.float64))
+    g.close()
+
+
 def test_torch_autograd_function_matches_adjoint(case):
     torch = pytest.importorskip("torch")
     from mpm_amd.autograd import Rollout
