@@ -383,6 +383,7 @@ int mpm_create_batch(const mpm_params* params, const int32_t* n_particles, int32
   alloc(&ctx->glist[1], sizeof(int) * nb);
   alloc(&ctx->gcnt, sizeof(int) * 4);
   alloc(&ctx->dstep, sizeof(int));
+  alloc(&ctx->bctr, sizeof(int));
   alloc(&ctx->gacc, gbytes);
   alloc(&ctx->gvel, gbytes);
   alloc(&ctx->gforce, gbytes);
@@ -401,6 +402,7 @@ int mpm_create_batch(const mpm_params* params, const int32_t* n_particles, int32
   launch_fill_int(ctx->gflag, ctx->nblocks, 0, ctx->stream);
   launch_fill_int(ctx->gcnt, 4, 0, ctx->stream);
   launch_fill_int(ctx->dstep, 1, 1, ctx->stream);
+  launch_fill_int(ctx->bctr, 1, 0, ctx->stream);
   launch_fill_int(ctx->nblk, 1, 0, ctx->stream);
   launch_clear(ctx->gacc, ctx->nnodes, ctx->stream);
   launch_clear(ctx->gvel, ctx->nnodes, ctx->stream);
@@ -426,7 +428,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   void* bufs[] = {ctx->P,       ctx->P2,      ctx->pid,      ctx->pid2,  ctx->keys,     ctx->vals,     ctx->keys2,
                   ctx->vals2,   ctx->hist,    ctx->blk_start, ctx->blk_end, ctx->blk_ids, ctx->blist,   ctx->nblk,
                   ctx->gflag,   ctx->glist[0], ctx->glist[1], ctx->gcnt,  ctx->dstep,    ctx->gacc,     ctx->gvel,
-                  ctx->gforce,  ctx->status,  ctx->dcount,  ctx->d_scenes, ctx->blk_scene};
+                  ctx->gforce,  ctx->status,  ctx->dcount,  ctx->d_scenes, ctx->blk_scene, ctx->bctr};
   for (void* b : bufs) (void)hipFree(b);
   for (DevCollider* c : ctx->scoll) (void)hipFree(c);
   for (float* c : ctx->adj_ckpt) (void)hipFree(c);
@@ -601,6 +603,7 @@ BlockArgs block_args(mpm_ctx* ctx, float dt) {
   ba.grid = ctx->block_grid;
   ba.rec = nullptr;
   ba.keep_acc = 0;
+  ba.bctr = ctx->bctr;
   return ba;
 }
 

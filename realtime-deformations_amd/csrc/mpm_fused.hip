@@ -226,6 +226,24 @@ __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S,
   }
 }
 
+// The next home block for this workgroup, claimed from the pass's counter
+// (home blocks differ in cost, so claiming balances the workgroups where a
+// fixed share would leave a tail); -1 when none is left.  Only workgroups
+// blockIdx.x < nblocks take part (the others return at once).  Each ends with
+// exactly one failed claim, and the one that makes the last of them (value
+// nblocks + participants - 1) re-zeroes the counter for the next launch.
+__device__ __forceinline__ int claim_block(int* ctr, int nblocks, int* slot) {
+  if (threadIdx.x == 0) {
+    const int participants = (int)gridDim.x < nblocks ? (int)gridDim.x : nblocks;
+    const int s = atomicAdd(ctr, 1);
+    if (s == nblocks + participants - 1) *ctr = 0;
+    *slot = s;
+  }
+  __syncthreads();
+  const int s = *slot;
+  return s < nblocks ? s : -1;
+}
+
 // Stress from F^n, F update F^n -> F^{n+1} with B, then the P2G scatter.
 template <int COMPOSE, int REC>
 __device__ __forceinline__ void p2g_one(const BlockArgs& ba, const Scene& S, int scene, double* __restrict__ ah,
@@ -305,10 +323,12 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 1) k_blo
   constexpr bool kP2G = MODE != MODE_G2P;
   __shared__ float vh[kG2P ? 3 * PLANE : 1];
   __shared__ double ah[kP2G ? 4 * PLANE : 1];
+  __shared__ int claimed;
   const KArgs& a = ba.a;
   const int t = threadIdx.x;
   const int nblocks = *ba.nblk;
-  for (int s = blockIdx.x; s < nblocks; s += gridDim.x) {
+  if ((int)blockIdx.x >= nblocks) return;
+  for (int s; (s = claim_block(ba.bctr, nblocks, &claimed)) >= 0;) {
     const int4 e = ba.blist[s];
     const int scene = e.w, start = e.y, end = e.z;
     const Scene S = BATCHED ? a.scenes[scene] : a.sc;  // by value: particle stores cannot alias it
@@ -416,11 +436,13 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
                                                        float* __restrict__ Gout, float4* __restrict__ gnode) {
   __shared__ float vh[3 * PLANE];
   __shared__ double gh[3 * PLANE];
+  __shared__ int claimed;
   const KArgs& a = ba.a;
   const int t = threadIdx.x;
   const int64_t sd = a.stride;
   const int nblocks = *ba.nblk;
-  for (int s = blockIdx.x; s < nblocks; s += gridDim.x) {
+  if ((int)blockIdx.x >= nblocks) return;
+  for (int s; (s = claim_block(ba.bctr, nblocks, &claimed)) >= 0;) {
     const int4 e = ba.blist[s];
     const int scene = e.w, start = e.y, end = e.z;
     const Scene S = BATCHED ? a.scenes[scene] : a.sc;

@@ -18,6 +18,7 @@ struct BlockArgs {
   const int* dstep;   // substep epoch (> 0)
   int64_t nblocks;    // grid blocks over all scenes
   int grid;           // workgroups of the persistent block pass
+  int* bctr;          // home-block claim counter (0 between launches)
   // reverse-mode recording (TEXTBOOK passes only): P2G and fused passes
   // write each particle's substep-start state (33 rows: x, v, B, F_E, F_P)
   // to rec, the G2P pass its new positions to rec's x rows; with keep_acc

@@ -113,6 +113,7 @@ struct mpm_ctx {
   int* glist[2] = {nullptr, nullptr};
   int* gcnt = nullptr;      // three rotating counters
   int* dstep = nullptr;     // device substep epoch (> 0)
+  int* bctr = nullptr;      // block passes' home-block claim counter
   int since_sort = -1;      // substeps since the last binning (-1: never binned)
   int resort_every = 16;
   int block_grid = 2048;    // workgroups of the persistent block pass
