@@ -139,25 +139,60 @@ __device__ __forceinline__ StencilInfo stencil_at(const Scene& S, const int o[3]
   return s;
 }
 
+// Per-lane traversal order of the 4x4x4 stencil in the LDS halo.  Lane l
+// visits the nodes of axis d starting at offset r[d] = (T[d] - lb[d]) mod 4,
+// T = (l >> 4, (l >> 2) & 3, l & 3), lb its stencil origin in the halo.  The
+// FP64 atomics of a wave issue as four 16-lane groups (FP32 reads as two
+// 32-lane groups); lanes of a group that share a cell, or a (cy, cz) column,
+// would otherwise hit one address or one bank pair at every step, which is
+// common in sparsely filled blocks (surfaces of a body).  Rotated, such lanes
+// differ in (y, z) mod 4 at every step and land on distinct addresses in
+// distinct bank pairs.  Only the order in which a particle sums its 64 nodes
+// changes.
+__device__ __forceinline__ void lane_rotation(const StencilInfo& s, int (&r)[3]) {
+  const int l = threadIdx.x & 63;
+  r[0] = ((l >> 4) - s.lb[0]) & 3;
+  r[1] = (((l >> 2) & 3) - s.lb[1]) & 3;
+  r[2] = ((l & 3) - s.lb[2]) & 3;
+}
+// w[k] <- w[(k + r) & 3]
+__device__ __forceinline__ void rot4(float (&w)[4], int r) {
+  const bool b1 = (r & 1) != 0, b2 = (r & 2) != 0;
+  const float a0 = b1 ? w[1] : w[0], a1 = b1 ? w[2] : w[1], a2 = b1 ? w[3] : w[2], a3 = b1 ? w[0] : w[3];
+  w[0] = b2 ? a2 : a0;
+  w[1] = b2 ? a3 : a1;
+  w[2] = b2 ? a0 : a2;
+  w[3] = b2 ? a1 : a3;
+}
+
 // G2P of one particle from the LDS halo, or from global memory when its
 // stencil left the halo.
 __device__ __forceinline__ void g2p_one(const KArgs& a, const Scene& S, const float* __restrict__ vh, const int o[3],
                                         F3 x, F3& v, M3& B) {
-  const StencilInfo s = stencil_at(S, o, x);
+  StencilInfo s = stencil_at(S, o, x);
   float dd[3][4];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const float xc = d == 0 ? x.x : (d == 1 ? x.y : x.z);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) dd[d][k] = (float)(s.ax[d].base - 1 + k) * S.h - xc;
-  }
   if (s.fits) {
+    int r[3];
+    lane_rotation(s, r);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float xc = d == 0 ? x.x : (d == 1 ? x.y : x.z);
+      rot4(s.ax[d].w, r[d]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) dd[d][k] = (float)(s.ax[d].base - 1 + ((k + r[d]) & 3)) * S.h - xc;
+    }
     const int q0 = s.lb[0] * SX + s.lb[1] * SY + s.lb[2];
     g2p_accum(s.ax, dd, [&](int ia, int ib, int ic) {
-      const int q = q0 + ia * SX + ib * SY + ic;
+      const int q = q0 + ((ia + r[0]) & 3) * SX + ((ib + r[1]) & 3) * SY + ((ic + r[2]) & 3);
       return f3(vh[q], vh[PLANE + q], vh[2 * PLANE + q]);
     }, v, B);
   } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float xc = d == 0 ? x.x : (d == 1 ? x.y : x.z);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) dd[d][k] = (float)(s.ax[d].base - 1 + k) * S.h - xc;
+    }
     g2p_accum(s.ax, dd, [&](int ia, int ib, int ic) {
       const float4 g = a.gvel[node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, S)];
       return f3(g.x, g.y, g.z);
@@ -167,26 +202,31 @@ __device__ __forceinline__ void g2p_one(const KArgs& a, const Scene& S, const fl
 
 // P2G scatter of mass and of m v + Q (x_i - x_p) over the 4x4x4 stencil
 // (cpp:99-117 and cpp:249-252 combined); `add(ia, ib, ic, m, px, py, pz)`.
+// `s.ax[d].w` hold the weights in traversal order: step k of axis d is node
+// base - 1 + ((k + r[d]) & 3) (r = 0: the plain order).
 template <typename Add>
-__device__ __forceinline__ void p2g_scatter(const Scene& S, const StencilInfo& s, F3 x, float m, F3 mv, const M3& Q,
-                                            Add add) {
+__device__ __forceinline__ void p2g_scatter(const Scene& S, const StencilInfo& s, const int (&r)[3], F3 x, float m,
+                                            F3 mv, const M3& Q, Add add) {
 #pragma unroll
   for (int ia = 0; ia < 4; ++ia) {
-    const float ddx = (float)(s.ax[0].base - 1 + ia) * S.h - x.x;
+    const int ja = (ia + r[0]) & 3;
+    const float ddx = (float)(s.ax[0].base - 1 + ja) * S.h - x.x;
     const float wa = s.ax[0].w[ia];
     const float qax = fmaf(Q.c[0][0], ddx, mv.x), qay = fmaf(Q.c[0][1], ddx, mv.y), qaz = fmaf(Q.c[0][2], ddx, mv.z);
 #pragma unroll
     for (int ib = 0; ib < 4; ++ib) {
-      const float ddy = (float)(s.ax[1].base - 1 + ib) * S.h - x.y;
+      const int jb = (ib + r[1]) & 3;
+      const float ddy = (float)(s.ax[1].base - 1 + jb) * S.h - x.y;
       const float wxy = wa * s.ax[1].w[ib];
       const float qx = fmaf(Q.c[1][0], ddy, qax);
       const float qy = fmaf(Q.c[1][1], ddy, qay);
       const float qz = fmaf(Q.c[1][2], ddy, qaz);
 #pragma unroll
       for (int ic = 0; ic < 4; ++ic) {
-        const float ddz = (float)(s.ax[2].base - 1 + ic) * S.h - x.z;
+        const int jc = (ic + r[2]) & 3;
+        const float ddz = (float)(s.ax[2].base - 1 + jc) * S.h - x.z;
         const float w = wxy * s.ax[2].w[ic];
-        add(ia, ib, ic, m * w, w * fmaf(Q.c[2][0], ddz, qx), w * fmaf(Q.c[2][1], ddz, qy),
+        add(ja, jb, jc, m * w, w * fmaf(Q.c[2][0], ddz, qx), w * fmaf(Q.c[2][1], ddz, qy),
             w * fmaf(Q.c[2][2], ddz, qz));
       }
     }
@@ -198,10 +238,14 @@ __device__ __forceinline__ void p2g_scatter(const Scene& S, const StencilInfo& s
 __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S, double* __restrict__ ah,
                                             const int o[3], F3 x, float m, F3 mv, const M3& Q) {
   const KArgs& a = ba.a;
-  const StencilInfo s = stencil_at(S, o, x);
+  StencilInfo s = stencil_at(S, o, x);
   if (s.fits) {
+    int r[3];
+    lane_rotation(s, r);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) rot4(s.ax[d].w, r[d]);
     const int q0 = s.lb[0] * SX + s.lb[1] * SY + s.lb[2];
-    p2g_scatter(S, s, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
+    p2g_scatter(S, s, r, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
       const int q = q0 + ia * SX + ib * SY + ic;
       atomicAdd(&ah[q], (double)vm);
       atomicAdd(&ah[PLANE + q], (double)px);
@@ -215,7 +259,8 @@ __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S,
         for (int dk = 0; dk < 2; ++dk)
           mark_block(ba, S, (s.ax[0].base - 1 + 3 * di) >> 2, (s.ax[1].base - 1 + 3 * dj) >> 2,
                      (s.ax[2].base - 1 + 3 * dk) >> 2);
-    p2g_scatter(S, s, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
+    const int r0[3] = {0, 0, 0};
+    p2g_scatter(S, s, r0, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
       float* g = reinterpret_cast<float*>(
           a.gacc + node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, S));
       atomicAdd(g + 0, vm);
