@@ -1,0 +1,36 @@
+// Block-pass building blocks shared by the forward passes (mpm_fused.hip)
+// and the reverse-mode block kernels (mpm_adjoint.hip): the LDS halo of a
+// 4x4x4-cell home block and the claiming of home blocks.
+#pragma once
+#include "mpm_fused.hpp"
+
+namespace mpm {
+namespace blk {
+
+// Halo of a home block: 9 nodes per axis (the block, the cubic stencil and
+// one cell of drift), strides (SX, SY, 1) chosen so a 4x4x4 cube of lanes
+// spreads over the LDS banks.
+constexpr int HN = 9;
+constexpr int SX = 112, SY = 12;
+constexpr int PLANE = 1008;  // >= 8*SX + 8*SY + 8 + 1
+
+// The next home block for this workgroup, claimed from the pass's counter
+// (home blocks differ in cost, so claiming balances the workgroups where a
+// fixed share would leave a tail); -1 when none is left.  Only workgroups
+// blockIdx.x < nblocks take part (the others return at once).  Each ends with
+// exactly one failed claim, and the one that makes the last of them (value
+// nblocks + participants - 1) re-zeroes the counter for the next launch.
+__device__ __forceinline__ int claim_block(int* ctr, int nblocks, int* slot) {
+  if (threadIdx.x == 0) {
+    const int participants = (int)gridDim.x < nblocks ? (int)gridDim.x : nblocks;
+    const int s = atomicAdd(ctr, 1);
+    if (s == nblocks + participants - 1) *ctr = 0;
+    *slot = s;
+  }
+  __syncthreads();
+  const int s = *slot;
+  return s < nblocks ? s : -1;
+}
+
+}  // namespace blk
+}  // namespace mpm

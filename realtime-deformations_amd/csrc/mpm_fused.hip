@@ -25,15 +25,17 @@
 // Every block-list entry carries its scene: one launch steps all scenes of a
 // context, each with its own grid spacing, grid, material and colliders read
 // from the device scene table.
-#include "mpm_fused.hpp"
+#include "mpm_block.hpp"
 
 namespace mpm {
 namespace {
 
 constexpr int TPB = 256;
-constexpr int HN = 9;
-constexpr int SX = 112, SY = 12;
-constexpr int PLANE = 1008;  // >= 8*SX + 8*SY + 8 + 1
+using blk::HN;
+using blk::SX;
+using blk::SY;
+using blk::PLANE;
+using blk::claim_block;
 
 enum Mode { MODE_P2G = 0, MODE_FUSED = 1, MODE_G2P = 2, MODE_MASS = 3 };
 
@@ -269,24 +271,6 @@ __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S,
       atomicAdd(g + 3, pz);
     });
   }
-}
-
-// The next home block for this workgroup, claimed from the pass's counter
-// (home blocks differ in cost, so claiming balances the workgroups where a
-// fixed share would leave a tail); -1 when none is left.  Only workgroups
-// blockIdx.x < nblocks take part (the others return at once).  Each ends with
-// exactly one failed claim, and the one that makes the last of them (value
-// nblocks + participants - 1) re-zeroes the counter for the next launch.
-__device__ __forceinline__ int claim_block(int* ctr, int nblocks, int* slot) {
-  if (threadIdx.x == 0) {
-    const int participants = (int)gridDim.x < nblocks ? (int)gridDim.x : nblocks;
-    const int s = atomicAdd(ctr, 1);
-    if (s == nblocks + participants - 1) *ctr = 0;
-    *slot = s;
-  }
-  __syncthreads();
-  const int s = *slot;
-  return s < nblocks ? s : -1;
 }
 
 // Stress from F^n, F update F^n -> F^{n+1} with B, then the P2G scatter.
@@ -542,8 +526,9 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
         for (int r = 0; r < 3; ++r) gB.c[c][r] = Gin[(int64_t)(FB + 3 * c + r) * sd + p];
       // one node: its velocity vi, weight w and weight gradient dw; returns
       // the node adjoint w (gv' + gB' d) and adds the position terms to gx
-      auto node = [&](int ia, int ib, int ic, F3 vi) {
-        const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib, k = ax[2].base - 1 + ic;
+      // (ia, ib, ic) index the weight arrays, (ja, jb, jc) the node offsets
+      auto node = [&](int ia, int ib, int ic, int ja, int jb, int jc, F3 vi) {
+        const int i = ax[0].base - 1 + ja, j = ax[1].base - 1 + jb, k = ax[2].base - 1 + jc;
         const float w = ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic];
         const F3 dw = f3(ax[0].dw[ia] * ax[1].w[ib] * ax[2].w[ic], ax[0].w[ia] * ax[1].dw[ib] * ax[2].w[ic],
                          ax[0].w[ia] * ax[1].w[ib] * ax[2].dw[ic]);
@@ -558,19 +543,32 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
         return u * w;
       };
       if (fits) {
+        // the block pass's per-lane traversal order (lane_rotation)
+        const int l_ = t & 63;
+        const int r0 = ((l_ >> 4) - lb[0]) & 3, r1 = (((l_ >> 2) & 3) - lb[1]) & 3, r2 = ((l_ & 3) - lb[2]) & 3;
+        rot4(ax[0].w, r0);
+        rot4(ax[0].dw, r0);
+        rot4(ax[1].w, r1);
+        rot4(ax[1].dw, r1);
+        rot4(ax[2].w, r2);
+        rot4(ax[2].dw, r2);
 #pragma unroll
         for (int ia = 0; ia < 4; ++ia) {
           __asm__ __volatile__("" ::: "memory");  // at most one x slab of nodes in flight
+          const int ja = (ia + r0) & 3;
 #pragma unroll
-          for (int ib = 0; ib < 4; ++ib)
+          for (int ib = 0; ib < 4; ++ib) {
+            const int jb = (ib + r1) & 3;
 #pragma unroll
             for (int ic = 0; ic < 4; ++ic) {
-              const int q = q0 + ia * SX + ib * SY + ic;
-              const F3 wu = node(ia, ib, ic, f3(vh[q], vh[PLANE + q], vh[2 * PLANE + q]));
+              const int jc = (ic + r2) & 3;
+              const int q = q0 + ja * SX + jb * SY + jc;
+              const F3 wu = node(ia, ib, ic, ja, jb, jc, f3(vh[q], vh[PLANE + q], vh[2 * PLANE + q]));
               atomicAdd(&gh[q], (double)wu.x);
               atomicAdd(&gh[PLANE + q], (double)wu.y);
               atomicAdd(&gh[2 * PLANE + q], (double)wu.z);
             }
+          }
         }
       } else {
         for (int ia = 0; ia < 4; ++ia)
@@ -578,7 +576,7 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
             for (int ic = 0; ic < 4; ++ic) {
               const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib, k = ax[2].base - 1 + ic;
               const int64_t g = node_index(i, j, k, S);
-              const F3 wu = node(ia, ib, ic, node_velocity(a.gacc[g], S, a.dt, i, j, k));
+              const F3 wu = node(ia, ib, ic, ia, ib, ic, node_velocity(a.gacc[g], S, a.dt, i, j, k));
               float* gn = reinterpret_cast<float*>(gnode + g);
               atomicAdd(gn + 1, wu.x);
               atomicAdd(gn + 2, wu.y);
