@@ -395,7 +395,9 @@ def bench_adjoint(args):
             "forward_ms_per_substep": fwd1_s / steps * 1e3,
             "backward_ms_per_substep": bwd1_s / steps * 1e3,
             "fwd_plus_bwd_over_plain_fwd": (fwd1_s + bwd1_s) / plain_s,
-            "record_bytes_per_substep": 132 * n + 16 * sum(int(np.prod(sc["dims"])) for sc in scs),
+            # state rows, room for every grid node's P2G sums, the block list
+            "record_bytes_per_substep": 132 * n + 16 * sum(int(np.prod(sc["dims"])) for sc in scs)
+            + 4 * (sum(int(np.prod(sc["dims"])) for sc in scs) // 64 + 1),
             "grad_rel_diff_vs_checkpoints": {"x0": rel(gx1, gx0), "v0": rel(gv1, gv0), "E0": rel(gE1, gE0)},
         },
         "grad_finite": bool(np.isfinite(gx0).all() and np.isfinite(gv0).all() and np.isfinite(gE0).all()),

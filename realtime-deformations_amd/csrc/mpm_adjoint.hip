@@ -15,7 +15,8 @@
 //                   node velocities from the substep's saved P2G sums, node
 //                   velocity adjoints and the position adjoint through the
 //                   weights
-//   k_adj_grid      wall mask and v = P/m + dt g: adjoints of (m_i, P_i)
+//   k_adj_grid      wall mask and v = P/m + dt g: adjoints of (m_i, P_i),
+//                   on the grid blocks the substep's record lists
 //   k_adj_p2g       P2G (gathered), stress, hardening and the F update:
 //                   adjoints of x, v, B, F_E, F_P of substep n and of E0
 // Derivatives of functions of singular values use the divided-difference
@@ -111,40 +112,57 @@ __device__ __forceinline__ bool svd_of(const M3& F, M3& Lb, float (&s)[3], M3& R
 
 // ------------------------------------------------------------------ kernels
 
-// wall mask and v = P/m + dt g: node adjoints (dL/dm, dL/dP) in place
-__global__ void __launch_bounds__(TPB) k_adj_grid(KArgs a, float4* gnode) {
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.nnodes) return;
-  const float4 acc = a.gacc[g];
-  float4 gn = gnode[g];
-  if (acc.x == 0.0f) {
-    gnode[g] = make_float4(0.f, 0.f, 0.f, 0.f);
-    return;
-  }
-  const Scene S = a.scenes[a.blk_scene[g >> 6]];
-  const F3 vpre = f3(acc.y / acc.x + a.dt * S.g[0], acc.z / acc.x + a.dt * S.g[1], acc.w / acc.x + a.dt * S.g[2]);
-  float mk[3] = {1.f, 1.f, 1.f};
-  if (S.bc_mode & MPM_BC_DOMAIN) {
-    const int64_t lb = (g >> 6) - S.blk0;
-    const int l = (int)(g & 63);
-    const int bk = (int)(lb % S.nb[2]);
-    const int64_t t2 = lb / S.nb[2];
-    const int bj = (int)(t2 % S.nb[1]), bi = (int)(t2 / S.nb[1]);
-    const int i = bi * 4 + (l >> 4), j = bj * 4 + ((l >> 2) & 3), k = bk * 4 + (l & 3);
-    // domain_walls: floor band sticky, other bands clamp the outward component
-    if (j < 3) {
-      mk[0] = mk[1] = mk[2] = 0.f;
-    } else {
-      if ((i < 3 && vpre.x < 0.f) || (i >= S.dims[0] - 3 && vpre.x > 0.f)) mk[0] = 0.f;
-      if (j >= S.dims[1] - 3 && vpre.y > 0.f) mk[1] = 0.f;
-      if ((k < 3 && vpre.z < 0.f) || (k >= S.dims[2] - 3 && vpre.z > 0.f)) mk[2] = 0.f;
+// wall mask and v = P/m + dt g: node adjoints (dL/dm, dL/dP) in place, on
+// the grid blocks of the substep's record (one wave per listed block; the
+// other nodes have no mass and keep a zero adjoint)
+__global__ void __launch_bounds__(TPB) k_adj_grid(KArgs a, GridRec r, float4* gnode) {
+  const int lane = threadIdx.x & 63;
+  const int cnt = r.list[0];
+  for (int s = blockIdx.x * 4 + (threadIdx.x >> 6); s < cnt; s += gridDim.x * 4) {
+    const int gb = r.list[1 + s];
+    const int64_t g = (int64_t)gb * 64 + lane;
+    const float4 acc = r.acc[(int64_t)s * 64 + lane];
+    const float4 gn = gnode[g];
+    if (acc.x == 0.0f) {
+      gnode[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
     }
+    const Scene S = a.scenes[a.blk_scene[gb]];
+    const F3 vpre = f3(acc.y / acc.x + a.dt * S.g[0], acc.z / acc.x + a.dt * S.g[1], acc.w / acc.x + a.dt * S.g[2]);
+    float mk[3] = {1.f, 1.f, 1.f};
+    if (S.bc_mode & MPM_BC_DOMAIN) {
+      const int64_t lb = gb - S.blk0;
+      const int bk = (int)(lb % S.nb[2]);
+      const int64_t t2 = lb / S.nb[2];
+      const int bj = (int)(t2 % S.nb[1]), bi = (int)(t2 / S.nb[1]);
+      const int i = bi * 4 + (lane >> 4), j = bj * 4 + ((lane >> 2) & 3), k = bk * 4 + (lane & 3);
+      // domain_walls: floor band sticky, other bands clamp the outward component
+      if (j < 3) {
+        mk[0] = mk[1] = mk[2] = 0.f;
+      } else {
+        if ((i < 3 && vpre.x < 0.f) || (i >= S.dims[0] - 3 && vpre.x > 0.f)) mk[0] = 0.f;
+        if (j >= S.dims[1] - 3 && vpre.y > 0.f) mk[1] = 0.f;
+        if ((k < 3 && vpre.z < 0.f) || (k >= S.dims[2] - 3 && vpre.z > 0.f)) mk[2] = 0.f;
+      }
+    }
+    const F3 gvp = f3(gn.y * mk[0], gn.z * mk[1], gn.w * mk[2]);
+    const float inv_m = 1.0f / acc.x;
+    const F3 gP = gvp * inv_m;
+    const float gm = -(acc.y * gvp.x + acc.z * gvp.y + acc.w * gvp.z) * inv_m * inv_m;
+    gnode[g] = make_float4(gm, gP.x, gP.y, gP.z);
   }
-  const F3 gvp = f3(gn.y * mk[0], gn.z * mk[1], gn.w * mk[2]);
-  const float inv_m = 1.0f / acc.x;
-  const F3 gP = gvp * inv_m;
-  const float gm = -(acc.y * gvp.x + acc.z * gvp.y + acc.w * gvp.z) * inv_m * inv_m;
-  gnode[g] = make_float4(gm, gP.x, gP.y, gP.z);
+}
+
+// set the record's block map (set = 1) before the substep is reversed, or
+// after it reset the map to -1 and zero the node adjoints of its blocks
+__global__ void __launch_bounds__(TPB) k_adj_map(GridRec r, int set, float4* gnode) {
+  const int lane = threadIdx.x & 63;
+  const int cnt = r.list[0];
+  for (int s = blockIdx.x * 4 + (threadIdx.x >> 6); s < cnt; s += gridDim.x * 4) {
+    const int gb = r.list[1 + s];
+    if (lane == 0) r.map[gb] = set ? s : -1;
+    if (!set) gnode[(int64_t)gb * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 }
 
 // P2G, stress, hardening and F-update adjoint for substep n.  St: state of
@@ -155,33 +173,39 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
   if (p >= a.n) return;
   const Scene S = a.scenes[scene_of_slot(a.scenes, a.nscenes, p)];
   const int64_t sd = a.stride;
-  const F3 x = f3(fld(St, sd, FX, p), fld(St, sd, FX + 1, p), fld(St, sd, FX + 2, p));
-  const F3 v = f3(fld(St, sd, FV, p), fld(St, sd, FV + 1, p), fld(St, sd, FV + 2, p));
-  const M3 B = ld_m3(St, sd, FB, p), FE = ld_m3(St, sd, FFE, p), FP = ld_m3(St, sd, FFP, p);
-  const float m = fld(a.P, sd, FM, p), V0 = fld(a.P, sd, FVOL, p), E = fld(a.P, sd, FE0, p);
   const float dinv = S.dinv;
   const M3 Dinv = dinv_matrix(dinv);
-  // forward stress pieces
   const float nu = S.lame.nu, xi = S.lame.xi;
-  const float mu0 = E / S.lame.dmu, lam0 = (E * nu) / S.lame.dlam;
-  const float JP = mdet(FP);
-  const float hard = expf(xi * (1.0f - JP));
-  const float mu = mu0 * hard, lam = lam0 * hard;
+  // Only the SVD of F_E, Q and the weights stay live across the gather (the
+  // other stress pieces are rebuilt from reloaded rows after it), so the
+  // gather can keep several node loads in flight.
   M3 Lb, Rb;
   float sv[3];
-  svd_of(FE, Lb, sv, Rb);
-  const M3 R = mmul(Lb, mtrans(Rb));
-  const float J = mdet(FE);
-  const M3 K = mtrans(minverse(FE));  // F_E^-T
-  const float cJ = (J - 1.0f) * J;
-  const M3 Pk = madd(mscale(msub(FE, R), 2.0f * mu), mscale(K, lam * cJ));
-  const M3 A = mmul(mscale(Dinv, V0), mmul(Pk, mtrans(FE)));
   M3 Q;
-  const M3 BD = mmul(B, Dinv);
+  F3 x, mv;
+  float m;
+  {
+    x = f3(fld(St, sd, FX, p), fld(St, sd, FX + 1, p), fld(St, sd, FX + 2, p));
+    const F3 v = f3(fld(St, sd, FV, p), fld(St, sd, FV + 1, p), fld(St, sd, FV + 2, p));
+    const M3 B = ld_m3(St, sd, FB, p), FE = ld_m3(St, sd, FFE, p), FP = ld_m3(St, sd, FFP, p);
+    m = fld(a.P, sd, FM, p);
+    const float V0 = fld(a.P, sd, FVOL, p), E = fld(a.P, sd, FE0, p);
+    const float hard = expf(xi * (1.0f - mdet(FP)));
+    const float mu = E / S.lame.dmu * hard, lam = (E * nu) / S.lame.dlam * hard;
+    svd_of(FE, Lb, sv, Rb);
+    const M3 R = mmul(Lb, mtrans(Rb));
+    const float J = mdet(FE);
+    const M3 K = mtrans(minverse(FE));  // F_E^-T
+    const float cJ = (J - 1.0f) * J;
+    const M3 Pk = madd(mscale(msub(FE, R), 2.0f * mu), mscale(K, lam * cJ));
+    const M3 A = mmul(mscale(Dinv, V0), mmul(Pk, mtrans(FE)));
+    const M3 BD = mmul(B, Dinv);
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int r = 0; r < 3; ++r) Q.c[c][r] = m * BD.c[c][r] - a.dt * A.c[c][r];
+      for (int r = 0; r < 3; ++r) Q.c[c][r] = m * BD.c[c][r] - a.dt * A.c[c][r];
+    mv = v * m;
+  }
   // gather the node adjoints
   AxisD ax[3];
   ax[0] = axis_d(x.x, S, S.dims[0] - 3);
@@ -190,15 +214,22 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
   F3 gv = f3(0.f, 0.f, 0.f);
   F3 gx = f3(fld(Gout, sd, AX, p), fld(Gout, sd, AX + 1, p), fld(Gout, sd, AX + 2, p));
   M3 gQ = mzero();
-  const F3 mv = v * m;
+#pragma unroll 1
   for (int ia = 0; ia < 4; ++ia)
-    for (int ib = 0; ib < 4; ++ib)
+#pragma unroll
+    for (int ib = 0; ib < 4; ++ib) {
+      __asm__ __volatile__("" ::: "memory");  // one z column of four node loads in flight
+      const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib;
+      float4 gcol[4];
+#pragma unroll
+      for (int ic = 0; ic < 4; ++ic) gcol[ic] = gnode[node_index(i, j, ax[2].base - 1 + ic, S)];
+#pragma unroll
       for (int ic = 0; ic < 4; ++ic) {
-        const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib, k = ax[2].base - 1 + ic;
+        const int k = ax[2].base - 1 + ic;
         const float w = ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic];
         const F3 dw = f3(ax[0].dw[ia] * ax[1].w[ib] * ax[2].w[ic], ax[0].w[ia] * ax[1].dw[ib] * ax[2].w[ic],
                          ax[0].w[ia] * ax[1].w[ib] * ax[2].dw[ic]);
-        const float4 gn = gnode[node_index(i, j, k, S)];
+        const float4 gn = gcol[ic];
         const F3 gP = f3(gn.y, gn.z, gn.w);
         const F3 d = f3((float)i * S.h - x.x, (float)j * S.h - x.y, (float)k * S.h - x.z);
         const F3 Qd = mvec(Q, d);
@@ -216,6 +247,21 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
 #pragma unroll
           for (int r = 0; r < 3; ++r) gQ.c[c][r] += w * gp[r] * dd[c];
       }
+    }
+  __asm__ __volatile__("" ::: "memory");  // reload the rows below rather than keep them live
+  // forward stress pieces, rebuilt
+  const M3 B = ld_m3(St, sd, FB, p), FE = ld_m3(St, sd, FFE, p), FP = ld_m3(St, sd, FFP, p);
+  const float V0 = fld(a.P, sd, FVOL, p), E = fld(a.P, sd, FE0, p);
+  const float mu0 = E / S.lame.dmu, lam0 = (E * nu) / S.lame.dlam;
+  const float JP = mdet(FP);
+  const float hard = expf(xi * (1.0f - JP));
+  const float mu = mu0 * hard, lam = lam0 * hard;
+  const M3 R = mmul(Lb, mtrans(Rb));
+  const float J = mdet(FE);
+  const M3 K = mtrans(minverse(FE));  // F_E^-T
+  const float cJ = (J - 1.0f) * J;
+  const M3 Pk = madd(mscale(msub(FE, R), 2.0f * mu), mscale(K, lam * cJ));
+  const M3 BD = mmul(B, Dinv);
   // Q = m B D - dt A
   M3 gB = mscale(mmul(gQ, mtrans(Dinv)), m);
   const M3 gA = mscale(gQ, -a.dt);
@@ -309,8 +355,13 @@ __global__ void __launch_bounds__(TPB) k_rows_to_aos(float* dst, const float* sr
 
 }  // namespace
 
-void launch_adj_grid(const KArgs& a, float4* gnode, hipStream_t s) {
-  k_adj_grid<<<nblk(a.nnodes), TPB, 0, s>>>(a, gnode);
+// one wave per listed grid block, up to 2048 workgroups of 4 waves
+inline int rec_grid(const KArgs& a) { return (int)std::max<int64_t>(1, std::min<int64_t>((a.nnodes / 64 + 3) / 4, 2048)); }
+void launch_adj_grid(const KArgs& a, const GridRec& r, float4* gnode, hipStream_t s) {
+  k_adj_grid<<<rec_grid(a), TPB, 0, s>>>(a, r, gnode);
+}
+void launch_adj_map(const KArgs& a, const GridRec& r, int set, float4* gnode, hipStream_t s) {
+  k_adj_map<<<rec_grid(a), TPB, 0, s>>>(r, set, gnode);
 }
 void launch_adj_p2g(const KArgs& a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE,
                     hipStream_t s) {

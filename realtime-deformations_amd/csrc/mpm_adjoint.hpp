@@ -1,6 +1,6 @@
 // Reverse-mode (adjoint) kernels: launch interface.
 #pragma once
-#include "mpm_internal.hpp"
+#include "mpm_fused.hpp"
 
 namespace mpm {
 
@@ -8,7 +8,10 @@ namespace mpm {
 // F_P), same layout and stride as the particle buffer.
 enum AdjField : int { AX = FX, AV = FV, AB = FB, AFE = FFE, AFP = FFP, NA = FM };
 
-void launch_adj_grid(const KArgs& a, float4* gnode, hipStream_t s);
+void launch_adj_grid(const KArgs& a, const GridRec& r, float4* gnode, hipStream_t s);
+// r.map from r.list (set = 1); or back to -1 with the listed blocks' node
+// adjoints zeroed (set = 0)
+void launch_adj_map(const KArgs& a, const GridRec& r, int set, float4* gnode, hipStream_t s);
 void launch_adj_p2g(const KArgs& a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE,
                     hipStream_t s);
 // row transfers: slot order (pid: caller index per slot) <-> caller order

@@ -434,7 +434,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   for (float* c : ctx->adj_ckpt) (void)hipFree(c);
   (void)hipFree(ctx->adj_const);
   for (auto& g : ctx->adj_segs) {
-    void* b[] = {g.st, g.gs, g.pid, g.blist, g.nblk};
+    void* b[] = {g.st, g.gs, g.gl, g.pid, g.blist, g.nblk};
     for (void* q : b) (void)hipFree(q);
   }
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
@@ -602,7 +602,7 @@ BlockArgs block_args(mpm_ctx* ctx, float dt) {
   ba.nblocks = ctx->nblocks;
   ba.grid = ctx->block_grid;
   ba.rec = nullptr;
-  ba.keep_acc = 0;
+  ba.grec = GridRec{nullptr, nullptr, nullptr};
   ba.bctr = ctx->bctr;
   return ba;
 }
@@ -1022,22 +1022,26 @@ int restore_rows(mpm_ctx* ctx, const float* rows33) {
   return MPM_OK;
 }
 
+// Grid record k of a segment: sums in gs (nnodes float4 per substep, room for
+// every grid block), block lists in gl (nblocks + 1 ints per substep).
+GridRec grid_record(const mpm_ctx* ctx, float4* gs, int* gl, int* map, int k) {
+  return GridRec{gs + (size_t)k * ctx->nnodes, gl + (size_t)k * (ctx->nblocks + 1), map};
+}
+
 // Run ns substeps from the current binning on the fused path (the first one
 // a P2G pass, then fused passes, then a final G2P pass: mpm_substeps(dt, ns)
 // without re-binning), the passes writing each substep's starting state
 // (rows 0..32 in slot order, seg + k * 33 rows) and the positions after the
-// last one (seg + ns * 33 rows), the P2G sums going straight into
-// gsave + k * nnodes and staying there.
-int record_segment(mpm_ctx* ctx, float dt, int ns, float* seg, float4* gsave) {
+// last one (seg + ns * 33 rows), and the grid pass recording the P2G sums of
+// each substep's active grid blocks (grid_record(k)).
+int record_segment(mpm_ctx* ctx, float dt, int ns, float* seg, float4* gs, int* gl) {
   const int64_t sd = ctx->stride;
   hipStream_t s = ctx->stream;
   if (!ctx->gacc_clean) reset_grid(ctx);
-  MPM_HIP(hipMemsetAsync(gsave, 0, sizeof(float4) * (size_t)ctx->nnodes * ns, s));
   for (int k = 0; k < ns; ++k) {
     BlockArgs ba = block_args(ctx, dt);
-    ba.a.gacc = gsave + (size_t)k * ctx->nnodes;
     ba.rec = seg + (size_t)k * NA * sd;
-    ba.keep_acc = 1;
+    ba.grec = grid_record(ctx, gs, gl, nullptr, k);
     launch_block_pass(ba, k == 0 ? PASS_P2G : PASS_FUSED, s);
     launch_grid_active(ba, s);
     launch_advance_step(ctx->dstep, ctx->gcnt, s);
@@ -1052,7 +1056,7 @@ int record_segment(mpm_ctx* ctx, float dt, int ns, float* seg, float4* gsave) {
 
 void free_segments(mpm_ctx* ctx) {
   for (auto& g : ctx->adj_segs) {
-    void* b[] = {g.st, g.gs, g.pid, g.blist, g.nblk};
+    void* b[] = {g.st, g.gs, g.gl, g.pid, g.blist, g.nblk};
     for (void* q : b) (void)hipFree(q);
   }
   ctx->adj_segs.clear();
@@ -1060,19 +1064,22 @@ void free_segments(mpm_ctx* ctx) {
 
 // Adjoint kernels over the ns recorded substeps of one segment, last to
 // first; Gs/Gs2 (slot order) swap per substep, Gs holds the result.
-void adjoint_segment(mpm_ctx* ctx, float dt, int ns, const float* seg, float4* gsave, float*& Gs, float*& Gs2,
-                     float4* gnode, float* gEs) {
+// gnode (zero on entry and exit outside the blocks being reversed) holds the
+// node adjoints; map: one int per grid block, -1 on entry and exit.
+void adjoint_segment(mpm_ctx* ctx, float dt, int ns, const float* seg, float4* gs, int* gl, int* map, float*& Gs,
+                     float*& Gs2, float4* gnode, float* gEs) {
   const int64_t sd = ctx->stride;
   hipStream_t s = ctx->stream;
   for (int k = ns - 1; k >= 0; --k) {
     const float* St = seg + (size_t)k * NA * sd;
     const float* Xn = seg + (size_t)(k + 1) * NA * sd;
     BlockArgs ba = block_args(ctx, dt);
-    ba.a.gacc = gsave + (size_t)k * ctx->nnodes;
-    launch_clear(gnode, ctx->nnodes, s);
+    ba.grec = grid_record(ctx, gs, gl, map, k);
+    launch_adj_map(ba.a, ba.grec, 1, gnode, s);
     launch_adj_g2p_block(ba, St, Xn, Gs, Gs2, gnode, s);
-    launch_adj_grid(ba.a, gnode, s);
+    launch_adj_grid(ba.a, ba.grec, gnode, s);
     launch_adj_p2g(ba.a, St, Gs, Gs2, gnode, gEs, s);
+    launch_adj_map(ba.a, ba.grec, 0, gnode, s);
     std::swap(Gs, Gs2);
   }
 }
@@ -1109,6 +1116,7 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
       g.ns = std::min(L, nsteps - done);
       bool ok = hipMalloc(&g.st, rows * (size_t)(g.ns + 1)) == hipSuccess &&
                 hipMalloc(&g.gs, sizeof(float4) * (size_t)ctx->nnodes * g.ns) == hipSuccess &&
+                hipMalloc(&g.gl, sizeof(int) * (size_t)(ctx->nblocks + 1) * g.ns) == hipSuccess &&
                 hipMalloc(&g.pid, sizeof(int) * (size_t)std::max(1, ctx->n)) == hipSuccess &&
                 hipMalloc(&g.blist, sizeof(int4) * (size_t)ctx->nblocks) == hipSuccess &&
                 hipMalloc(&g.nblk, sizeof(int)) == hipSuccess;
@@ -1122,7 +1130,7 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
       MPM_HIP(hipMemcpyAsync(g.pid, ctx->pid, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
       MPM_HIP(hipMemcpyAsync(g.blist, ctx->blist, sizeof(int4) * ctx->nblocks, hipMemcpyDeviceToDevice, ctx->stream));
       MPM_HIP(hipMemcpyAsync(g.nblk, ctx->nblk, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
-      if (int rc = record_segment(ctx, dt, g.ns, g.st, g.gs)) return rc;
+      if (int rc = record_segment(ctx, dt, g.ns, g.st, g.gs, g.gl)) return rc;
       done += g.ns;
     }
     ctx->gacc_clean = false;
@@ -1147,6 +1155,7 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   hipStream_t s = ctx->stream;
   float *Gc = nullptr, *Gs = nullptr, *Gs2 = nullptr, *gEs = nullptr, *gEc = nullptr, *seg = nullptr, *tmp = nullptr;
   float4 *gnode = nullptr, *gsave = nullptr;
+  int *glist = nullptr, *gmap = nullptr;
   const size_t rows = sizeof(float) * NA * sd;
   bool ok = true;
   auto alloc = [&](auto** ptr, size_t bytes) {
@@ -1162,12 +1171,14 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   if (every > 1) {
     alloc(&seg, rows * (size_t)(every + 1));
     alloc(&gsave, sizeof(float4) * (size_t)ctx->nnodes * every);
+    alloc(&glist, sizeof(int) * (size_t)(ctx->nblocks + 1) * every);
   }
+  alloc(&gmap, sizeof(int) * (size_t)ctx->nblocks);
   alloc(&tmp, sizeof(float) * 3 * (size_t)std::max(1, n));
   alloc(&gnode, sizeof(float4) * (size_t)ctx->nnodes);
   int rc = MPM_OK;
   auto cleanup = [&] {
-    void* b[] = {Gc, Gs, Gs2, gEs, gEc, seg, tmp, gnode, gsave};
+    void* b[] = {Gc, Gs, Gs2, gEs, gEc, seg, tmp, gnode, gsave, glist, gmap};
     for (void* p : b) (void)hipFree(p);
   };
   if (!ok) {
@@ -1177,6 +1188,8 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   }
   MPM_HIP(hipMemsetAsync(Gc, 0, rows, s));
   MPM_HIP(hipMemsetAsync(gEc, 0, sizeof(float) * sd, s));
+  launch_clear(gnode, ctx->nnodes, s);
+  launch_fill_int(gmap, ctx->nblocks, -1, s);
   if (gx) {
     MPM_HIP(hipMemcpyAsync(tmp, gx, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
     launch_aos_to_rows(Gc + (int64_t)AX * sd, tmp, n, 3, sd, s);
@@ -1199,7 +1212,7 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
       launch_rows_to_slots(ctx->P + (int64_t)FM * sd, ctx->adj_const, ctx->pid, sd, n, 3, s);
       launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
       MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
-      adjoint_segment(ctx, dt, g.ns, g.st, g.gs, Gs, Gs2, gnode, gEs);
+      adjoint_segment(ctx, dt, g.ns, g.st, g.gs, g.gl, gmap, Gs, Gs2, gnode, gEs);
       launch_rows_to_caller(Gc, Gs, ctx->pid, sd, n, NA, s);
       launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
     }
@@ -1209,10 +1222,10 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
     const int ns = std::min(every, steps - c * every);
     if ((rc = restore_rows(ctx, ctx->adj_ckpt[c])) || (rc = rebin(ctx))) break;
     // re-run the segment, keeping each substep's starting state and P2G sums
-    if ((rc = record_segment(ctx, dt, ns, seg, gsave))) break;
+    if ((rc = record_segment(ctx, dt, ns, seg, gsave, glist))) break;
     launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
     MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
-    adjoint_segment(ctx, dt, ns, seg, gsave, Gs, Gs2, gnode, gEs);
+    adjoint_segment(ctx, dt, ns, seg, gsave, glist, gmap, Gs, Gs2, gnode, gEs);
     launch_rows_to_caller(Gc, Gs, ctx->pid, sd, n, NA, s);
     launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
     ctx->gacc_clean = false;

@@ -486,7 +486,7 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
       const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
       F3 v = f3(0.f, 0.f, 0.f);
       if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2])
-        v = node_velocity(a.gacc[node_index(gi, gj, gk, S)], S, a.dt, gi, gj, gk);
+        v = node_velocity(rec_acc(ba.grec, node_index(gi, gj, gk, S)), S, a.dt, gi, gj, gk);
       const int l = hx * SX + hy * SY + hz;
       vh[l] = v.x;
       vh[PLANE + l] = v.y;
@@ -576,11 +576,16 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
             for (int ic = 0; ic < 4; ++ic) {
               const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib, k = ax[2].base - 1 + ic;
               const int64_t g = node_index(i, j, k, S);
-              const F3 wu = node(ia, ib, ic, ia, ib, ic, node_velocity(a.gacc[g], S, a.dt, i, j, k));
-              float* gn = reinterpret_cast<float*>(gnode + g);
-              atomicAdd(gn + 1, wu.x);
-              atomicAdd(gn + 2, wu.y);
-              atomicAdd(gn + 3, wu.z);
+              const int c = ba.grec.map[g >> 6];
+              const float4 acc = c < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : ba.grec.acc[(int64_t)c * 64 + (g & 63)];
+              const F3 wu = node(ia, ib, ic, ia, ib, ic, node_velocity(acc, S, a.dt, i, j, k));
+              // massless nodes (outside the listed blocks) carry no adjoint
+              if (c >= 0) {
+                float* gn = reinterpret_cast<float*>(gnode + g);
+                atomicAdd(gn + 1, wu.x);
+                atomicAdd(gn + 2, wu.y);
+                atomicAdd(gn + 3, wu.z);
+              }
             }
       }
 #pragma unroll
@@ -593,8 +598,10 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
       const float u0 = (float)gh[l], u1 = (float)gh[PLANE + l], u2 = (float)gh[2 * PLANE + l];
       if (u0 != 0.0f || u1 != 0.0f || u2 != 0.0f) {
         const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
-        if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2]) {
-          float* gn = reinterpret_cast<float*>(gnode + node_index(gi, gj, gk, S));
+        const int64_t g = node_index(gi, gj, gk, S);
+        if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2] &&
+            ba.grec.map[g >> 6] >= 0) {
+          float* gn = reinterpret_cast<float*>(gnode + g);
           atomicAdd(gn + 1, u0);
           atomicAdd(gn + 2, u1);
           atomicAdd(gn + 3, u2);
@@ -616,11 +623,16 @@ template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(Bloc
   const int* glist_prev = ba.glist[(step + 1) & 1];
   const int ncur = ba.gcnt[step % 3];
   const int nprev = ba.gcnt[(step + 2) % 3];
+  if (ba.grec.acc && blockIdx.x == 0 && threadIdx.x == 0) ba.grec.list[0] = ncur;
   for (int s = blockIdx.x * 4 + wave; s < ncur + nprev; s += gridDim.x * 4) {
     if (s < ncur) {
       const int gb = glist_cur[s];
       const int64_t g = (int64_t)gb * 64 + lane;
       const float4 acc = a.gacc[g];
+      if (ba.grec.acc) {
+        ba.grec.acc[(int64_t)s * 64 + lane] = acc;
+        if (lane == 0) ba.grec.list[1 + s] = gb;
+      }
       float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
       if (acc.x != 0.0f) {
         const Scene S = BATCHED ? a.scenes[a.blk_scene[gb]] : a.sc;
@@ -638,7 +650,7 @@ template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(Bloc
         }
         out = make_float4(v.x, v.y, v.z, acc.x);
       }
-      if (!ba.keep_acc && (acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f || acc.w != 0.0f))
+      if (acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f || acc.w != 0.0f)
         a.gacc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
       a.gvel[g] = out;
     } else {

@@ -8,6 +8,23 @@ namespace mpm {
 // launch_advance_step after each grid pass), so one captured substep can be
 // replayed as a HIP graph: grid-block lists alternate with the epoch's
 // parity, their counters rotate with it modulo 3.
+// Reverse-mode record of one substep's grid: the P2G sums of its active
+// grid blocks, 64 nodes each, in the order of list (list[0]: the count,
+// list[1 + s]: the grid block of record slot s).  While the substep is
+// reversed, map (one entry per grid block, -1 for blocks not listed) holds
+// each listed block's slot.
+struct GridRec {
+  float4* acc;
+  int* list;
+  int* map;
+};
+
+// P2G sums of node g from a grid record (zero outside the listed blocks)
+__device__ __forceinline__ float4 rec_acc(const GridRec& r, int64_t g) {
+  const int c = r.map[g >> 6];
+  return c < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : r.acc[(int64_t)c * 64 + (g & 63)];
+}
+
 struct BlockArgs {
   KArgs a;
   const int4* blist;  // (home block, start, end, 0)
@@ -21,21 +38,21 @@ struct BlockArgs {
   int* bctr;          // home-block claim counter (0 between launches)
   // reverse-mode recording (TEXTBOOK passes only): P2G and fused passes
   // write each particle's substep-start state (33 rows: x, v, B, F_E, F_P)
-  // to rec, the G2P pass its new positions to rec's x rows; with keep_acc
-  // the grid pass leaves the P2G sums in a.gacc instead of clearing them
+  // to rec, the G2P pass its new positions to rec's x rows; with grec.acc
+  // set the grid pass records the substep's P2G sums there
   float* rec;
-  int keep_acc;
+  GridRec grec;
 };
 
 enum { PASS_P2G = 0, PASS_FUSED = 1, PASS_G2P = 2, PASS_MASS = 3 };
 
 // Reverse mode's advection + G2P adjoint on the block structure: per block,
-// node velocities (from the substep's P2G sums in ba.a.gacc) in an LDS halo
-// and the node-velocity adjoints (w (gv' + gB' d)) accumulated in an FP64 LDS
-// halo, flushed with one set of global atomics per block.  St: stored state
-// of the substep (slot order); Xn: rows holding the next state's positions;
-// Gin: the adjoints of the next state; Gout.x receives the partial position
-// adjoint; gnode.yzw the node adjoints.
+// node velocities (from the substep's grid record ba.grec, its map set) in an
+// LDS halo and the node-velocity adjoints (w (gv' + gB' d)) accumulated in an
+// FP64 LDS halo, flushed with one set of global atomics per block into the
+// listed grid blocks.  St: stored state of the substep (slot order); Xn: rows
+// holding the next state's positions; Gin: the adjoints of the next state;
+// Gout.x receives the partial position adjoint; gnode.yzw the node adjoints.
 void launch_adj_g2p_block(const BlockArgs& ba, const float* St, const float* Xn, const float* Gin, float* Gout,
                           float4* gnode, hipStream_t s);
 void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s);

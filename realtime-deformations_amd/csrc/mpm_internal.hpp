@@ -155,7 +155,8 @@ struct mpm_ctx {
   struct AdjSegment {
     int ns = 0;
     float* st = nullptr;     // (ns + 1) x 33 rows
-    float4* gs = nullptr;    // ns x nnodes
+    float4* gs = nullptr;    // ns x nnodes: P2G sums of the active grid blocks
+    int* gl = nullptr;       // ns x (nblocks + 1): their count and list
     int* pid = nullptr;      // n
     int4* blist = nullptr;   // nblocks
     int* nblk = nullptr;     // 1
