@@ -283,6 +283,13 @@ int mpm_copy_probe(mpm_ctx* ctx, int64_t bytes);
  * reorder, the caller index held by every slot.  Any output may be NULL. */
 int mpm_debug_bin(mpm_ctx* ctx, uint32_t* sorted_keys, uint32_t* perm, int32_t* slot_caller_index);
 
+/* Home-block list of the last binning (no reference counterpart), in the
+ * order the block passes claim it: per block (grid block over all scenes,
+ * first slot, end slot, scene), 4 ints each.  *count receives the number of
+ * blocks; at most max_blocks are written (blocks may be NULL to query the
+ * count). */
+int mpm_debug_blocks(mpm_ctx* ctx, int32_t* blocks, int32_t max_blocks, int32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -323,18 +323,31 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
   st_m3(Gout, sd, AFP, p, gFP);
 }
 
+// element di of `rows` rows of dst <- element si of those of src; eight
+// loads in flight before their stores
+__device__ __forceinline__ void copy_rows(float* __restrict__ dst, int64_t di, const float* __restrict__ src, int64_t si,
+                                          int64_t stride, int rows) {
+  int f = 0;
+  for (; f + 8 <= rows; f += 8) {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = src[(int64_t)(f + k) * stride + si];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[(int64_t)(f + k) * stride + di] = v[k];
+  }
+  for (; f < rows; ++f) dst[(int64_t)f * stride + di] = src[(int64_t)f * stride + si];
+}
+
 // slot <-> caller order of `rows` consecutive rows
 __global__ void __launch_bounds__(TPB) k_rows_to_slots(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const int q = pid[p];
-  for (int f = 0; f < rows; ++f) dst[(int64_t)f * stride + p] = src[(int64_t)f * stride + q];
+  copy_rows(dst, p, src, pid[p], stride, rows);
 }
 __global__ void __launch_bounds__(TPB) k_rows_to_caller(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const int q = pid[p];
-  for (int f = 0; f < rows; ++f) dst[(int64_t)f * stride + q] = src[(int64_t)f * stride + p];
+  copy_rows(dst, pid[p], src, p, stride, rows);
 }
 __global__ void __launch_bounds__(TPB) k_add_to_caller(float* dst, const float* src, const int* pid, int n) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;

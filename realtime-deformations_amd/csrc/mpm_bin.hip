@@ -21,7 +21,6 @@ constexpr int RS_TILE = RS_T * RS_ITEMS;
 
 __global__ void __launch_bounds__(256) k_keys(BinArgs b) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) *b.nblk = 0;
   if (p >= b.n) return;
   const Scene S = b.scenes[scene_of_slot(b.scenes, b.nscenes, p)];
   int c[3];
@@ -197,16 +196,95 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t* keys, const
 }
 
 // First and last sorted index of every non-empty block; compact block list.
-__global__ void __launch_bounds__(256) k_block_detect(BinArgs b) {
+// Block boundaries of the sorted keys: blk_start / blk_end per grid block,
+// and the number of blocks that start in each workgroup (wgcnt).
+__global__ void __launch_bounds__(256) k_block_detect(BinArgs b, uint32_t* wgcnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b.n) return;
-  const uint32_t blk = b.keys[i] >> 6;
-  if (i == 0 || (b.keys[i - 1] >> 6) != blk) {
-    b.blk_start[blk] = i;
-    const int slot = atomicAdd(b.nblk, 1);
-    b.blk_ids[slot] = (int)blk;
+  bool start = false;
+  if (i < b.n) {
+    const uint32_t blk = b.keys[i] >> 6;
+    start = i == 0 || (b.keys[i - 1] >> 6) != blk;
+    if (start) b.blk_start[blk] = i;
+    if (i == b.n - 1 || (b.keys[i + 1] >> 6) != blk) b.blk_end[blk] = i + 1;
   }
-  if (i == b.n - 1 || (b.keys[i + 1] >> 6) != blk) b.blk_end[blk] = i + 1;
+  const int c = __syncthreads_count(start);
+  if (threadIdx.x == 0) wgcnt[blockIdx.x] = (uint32_t)c;
+}
+
+// Exclusive scan of the per-workgroup block counts in place (one workgroup,
+// BS_ITEMS consecutive counts per thread) and their total into *nblk.
+constexpr int BS_T = 1024, BS_ITEMS = 8;
+__global__ void __launch_bounds__(BS_T) k_block_scan(uint32_t* wgcnt, int nwg, int* nblk) {
+  __shared__ uint32_t wsum[BS_T / 64];
+  __shared__ uint32_t carry;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (t == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nwg; base += BS_T * BS_ITEMS) {
+    uint32_t v[BS_ITEMS], sum = 0;
+#pragma unroll
+    for (int k = 0; k < BS_ITEMS; ++k) {
+      const int i = base + t * BS_ITEMS + k;
+      v[k] = i < nwg ? wgcnt[i] : 0u;
+      sum += v[k];
+    }
+    uint32_t x = sum;  // inclusive scan over the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (t < 64) {
+      uint32_t w = t < BS_T / 64 ? wsum[t] : 0u;
+#pragma unroll
+      for (int off = 1; off < BS_T / 64; off <<= 1) {
+        const uint32_t y = __shfl_up(w, off, 64);
+        if (lane >= off) w += y;
+      }
+      if (t < BS_T / 64) wsum[t] = w;
+    }
+    __syncthreads();
+    uint32_t run = carry + (wave ? wsum[wave - 1] : 0u) + x - sum;
+#pragma unroll
+    for (int k = 0; k < BS_ITEMS; ++k) {
+      const int i = base + t * BS_ITEMS + k;
+      if (i < nwg) wgcnt[i] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (t == 0) carry += wsum[BS_T / 64 - 1];
+    __syncthreads();
+  }
+  if (t == 0) *nblk = (int)carry;
+}
+
+// The block passes claim home blocks in list order.  Rank r in key order
+// goes to list slot (r * CLAIM_STRIDE) mod nblk, so that blocks claimed at
+// the same time lie apart and do not flush into the same grid nodes at once.
+constexpr uint32_t CLAIM_STRIDE = 7919, CLAIM_STRIDE_ALT = 7907;  // primes
+__device__ __forceinline__ uint32_t claim_slot(uint32_t r, uint32_t nblk) {
+  const uint64_t m = nblk % CLAIM_STRIDE ? CLAIM_STRIDE : CLAIM_STRIDE_ALT;  // coprime with nblk
+  return (uint32_t)(((uint64_t)r * m) % nblk);
+}
+
+// Compact list of the non-empty blocks: the block starting in workgroup w
+// has rank wgoff[w] + its rank among that workgroup's starts in key order.
+__global__ void __launch_bounds__(256) k_block_ids(BinArgs b, const uint32_t* wgoff) {
+  __shared__ uint32_t wcnt[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool start = i < b.n && (i == 0 || (b.keys[i - 1] >> 6) != (b.keys[i] >> 6));
+  const unsigned long long m = __ballot(start);
+  if (lane == 0) wcnt[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (start) {
+    uint32_t r = wgoff[blockIdx.x];
+    for (int w = 0; w < wave; ++w) r += wcnt[w];
+    r += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    b.blk_ids[claim_slot(r, (uint32_t)*b.nblk)] = (int)(b.keys[i] >> 6);
+  }
 }
 
 // Position of the k-th (0-based) set bit of m.
@@ -255,7 +333,22 @@ __device__ __forceinline__ int wave_max(int v) {
   for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
   return v;
 }
-__global__ void __launch_bounds__(256) k_reorder(BinArgs b) {
+// Particle fields and caller index from slot src (current order) to slot
+// dst (binned order); all loads are issued before the first store.
+__device__ __forceinline__ void move_particle(const BinArgs& b, int src, int dst) {
+  const float* __restrict__ P = b.P;
+  float* __restrict__ Pout = b.Pout;
+  float v[NFIELDS];
+#pragma unroll
+  for (int f = 0; f < NFIELDS; ++f) v[f] = P[(int64_t)f * b.stride + src];
+  const int id = b.pid[src];
+#pragma unroll
+  for (int f = 0; f < NFIELDS; ++f) Pout[(int64_t)f * b.stride + dst] = v[f];
+  b.pid_out[dst] = id;
+}
+
+constexpr int RO_T = 256;
+__global__ void __launch_bounds__(RO_T) k_reorder(BinArgs b) {
   __shared__ int cnt[64];
   __shared__ int cstart[64];
   __shared__ unsigned char dmask[16][RR_MAX];
@@ -264,13 +357,15 @@ __global__ void __launch_bounds__(256) k_reorder(BinArgs b) {
   __shared__ unsigned short gptr[GMAX][16];
   __shared__ int maxc_s, table_s;
   const int t = threadIdx.x;
-  for (int s = blockIdx.x; s < *b.nblk; s += gridDim.x) {
+  const int nblk = *b.nblk;
+  for (int r = blockIdx.x; r < nblk; r += gridDim.x) {  // blocks in key order
+    const int s = (int)claim_slot((uint32_t)r, (uint32_t)nblk);
     const int blk = b.blk_ids[s];
     const int start = b.blk_start[blk], end = b.blk_end[blk];
     const int nb = end - start;
     if (t < 64) cnt[t] = 0;
     __syncthreads();
-    for (int i = start + t; i < end; i += 256) atomicAdd(&cnt[b.keys[i] & 63u], 1);
+    for (int i = start + t; i < end; i += RO_T) atomicAdd(&cnt[b.keys[i] & 63u], 1);
     __syncthreads();
     if (t < 64) {
       const int c = cnt[t];
@@ -330,7 +425,7 @@ __global__ void __launch_bounds__(256) k_reorder(BinArgs b) {
     __syncthreads();
     if (table_s) {
       const int maxc = maxc_s;
-      for (int dst = start + t; dst < end; dst += 256) {
+      for (int dst = start + t; dst < end; dst += RO_T) {
         const int j = dst - start, gi = j >> 4, r = j & 15;
         int g = 15;
         while (gofs[gi][g] > r) --g;
@@ -343,12 +438,10 @@ __global__ void __launch_bounds__(256) k_reorder(BinArgs b) {
         }
         const int cx = select_bit((unsigned long long)dmask[g][lo], p - dbase[g][lo]);
         const int c = (cx << 4) | g;
-        const int src = (int)b.vals[start + cstart[c] + lo];
-        for (int f = 0; f < NFIELDS; ++f) b.Pout[(int64_t)f * b.stride + dst] = b.P[(int64_t)f * b.stride + src];
-        b.pid_out[dst] = b.pid[src];
+        move_particle(b, (int)b.vals[start + cstart[c] + lo], dst);
       }
     } else {
-      for (int i = start + t; i < end; i += 256) {
+      for (int i = start + t; i < end; i += RO_T) {
         const int c = (int)(b.keys[i] & 63u);
         const int r = i - start - cstart[c];
         int pos = 0;
@@ -356,10 +449,7 @@ __global__ void __launch_bounds__(256) k_reorder(BinArgs b) {
           const int k = cnt[cc];
           pos += (k < r ? k : r) + ((cc < c && k > r) ? 1 : 0);
         }
-        const int dst = start + pos;
-        const int src = (int)b.vals[i];
-        for (int f = 0; f < NFIELDS; ++f) b.Pout[(int64_t)f * b.stride + dst] = b.P[(int64_t)f * b.stride + src];
-        b.pid_out[dst] = b.pid[src];
+        move_particle(b, (int)b.vals[i], start + pos);
       }
     }
     if (t == 0) b.blist[s] = make_int4(blk, start, end, b.blk_scene[blk]);
@@ -393,9 +483,13 @@ void launch_bin(BinArgs b, hipStream_t s) {
   BinArgs c = b;
   c.keys = k0;
   c.vals = v0;
-  k_block_detect<<<nb, 256, 0, s>>>(c);
+  // per-workgroup block counts in the sort's histogram space (free after the
+  // sort; 256 * ntiles >= nb)
+  k_block_detect<<<nb, 256, 0, s>>>(c, b.hist);
+  k_block_scan<<<1, BS_T, 0, s>>>(b.hist, nb, b.nblk);
+  k_block_ids<<<nb, 256, 0, s>>>(c, b.hist);
   const int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (int64_t)b.n / 64), 4096);
-  k_reorder<<<grid, 256, 0, s>>>(c);
+  k_reorder<<<grid, RO_T, 0, s>>>(c);
 }
 
 // Keys and permutation only (the test hook for bit-exact sort parity).

@@ -1271,6 +1271,19 @@ extern "C" int mpm_debug_bin(mpm_ctx* ctx, uint32_t* sorted_keys, uint32_t* perm
   return MPM_OK;
 }
 
+extern "C" int mpm_debug_blocks(mpm_ctx* ctx, int32_t* blocks, int32_t max_blocks, int32_t* count) {
+  if (check_ctx(ctx) || !count || max_blocks < 0) return MPM_ERR_ARG;
+  int nb = 0;
+  if (ctx->n > 0 && ctx->since_sort >= 0) {
+    MPM_HIP(hipMemcpyAsync(&nb, ctx->nblk, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    MPM_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  *count = nb;
+  if (blocks && nb > 0)
+    MPM_HIP(hipMemcpy(blocks, ctx->blist, sizeof(int4) * (size_t)std::min(nb, max_blocks), hipMemcpyDeviceToHost));
+  return MPM_OK;
+}
+
 namespace {
 __global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
                                                     int64_t n) {

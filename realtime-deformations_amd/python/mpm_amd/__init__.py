@@ -86,7 +86,7 @@ EXPORTS = [
     "mpm_default_params", "mpm_create", "mpm_destroy", "mpm_last_error", "mpm_num_particles",
     "mpm_set_particles", "mpm_set_state", "mpm_set_colliders", "mpm_init_volumes", "mpm_stage",
     "mpm_substeps", "mpm_sync", "mpm_get_particles", "mpm_get_grid", "mpm_profile_begin", "mpm_profile_end",
-    "mpm_count_active", "mpm_debug_bin", "mpm_copy_probe", "mpm_use_graphs",
+    "mpm_count_active", "mpm_debug_bin", "mpm_debug_blocks", "mpm_copy_probe", "mpm_use_graphs",
     "mpm_create_batch", "mpm_num_scenes", "mpm_scene_set_particles", "mpm_scene_set_state",
     "mpm_scene_set_colliders", "mpm_scene_get_particles", "mpm_scene_get_grid", "mpm_scene_status",
     "mpm_set_tuning", "mpm_get_tuning", "mpm_autotune", "mpm_adjoint_forward", "mpm_adjoint_backward",
@@ -138,6 +138,7 @@ def lib():
         L.mpm_profile_end.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.mpm_count_active.argtypes = [vp, C.POINTER(C.c_int64)]
         L.mpm_debug_bin.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]
+        L.mpm_debug_blocks.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]
         L.mpm_copy_probe.argtypes = [vp, C.c_int64]
         L.mpm_scene_generate.argtypes = [vp, C.c_int32, C.POINTER(Generator), C.POINTER(C.c_int64)]
         L.mpm_generate.argtypes = [vp, C.POINTER(Generator), C.POINTER(C.c_int64)]
@@ -297,6 +298,17 @@ class Simulation:
                                          p.ctypes.data_as(C.POINTER(C.c_uint32)),
                                          s.ctypes.data_as(C.POINTER(C.c_int32))))
         return k, p, s
+
+    def debug_blocks(self):
+        """-> (nblocks, 4) int32: home blocks of the last binning in claim
+        order (grid block, first slot, end slot, scene)"""
+        cnt = C.c_int32(0)
+        self._check(self.L.mpm_debug_blocks(self.ctx, None, 0, C.byref(cnt)))
+        out = np.zeros((cnt.value, 4), np.int32)
+        if cnt.value:
+            self._check(self.L.mpm_debug_blocks(self.ctx, out.ctypes.data_as(C.POINTER(C.c_int32)), cnt.value,
+                                                C.byref(cnt)))
+        return out
 
     def particles(self):
         n = self.n

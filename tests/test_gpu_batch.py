@@ -137,3 +137,26 @@ def test_autotune_keeps_state_and_picks_a_candidate():
     b.substeps(1e-4, 10)
     b.sync()
     b.close()
+
+
+def test_batch_block_list_covers_each_scene():
+    """After a batched binning every listed home block lies in its scene's
+    grid blocks and its slot range in that scene's particle slots, and the
+    listed ranges tile all slots exactly once."""
+    scs = _mixed_batch()
+    b, _ = _run_batch(scs, 1e-4, 3)
+    b.debug_bin()
+    blocks = b.debug_blocks()
+    nb = [int(np.prod([(d + 3) // 4 for d in s["dims"]])) for s in scs]
+    blk0 = np.concatenate([[0], np.cumsum(nb)])
+    ns = [s["x"].shape[0] for s in scs]
+    p0 = np.concatenate([[0], np.cumsum(ns)])
+    sc = blocks[:, 3]
+    assert sc.min() >= 0 and sc.max() < len(scs)
+    assert np.all((blocks[:, 0] >= blk0[sc]) & (blocks[:, 0] < blk0[sc + 1]))
+    assert np.all((blocks[:, 1] >= p0[sc]) & (blocks[:, 2] <= p0[sc + 1]) & (blocks[:, 1] < blocks[:, 2]))
+    assert np.unique(blocks[:, 0]).size == blocks.shape[0]
+    r = blocks[np.argsort(blocks[:, 1])]
+    assert r[0, 1] == 0 and r[-1, 2] == p0[-1]
+    np.testing.assert_array_equal(r[1:, 1], r[:-1, 2])
+    b.close()

@@ -171,6 +171,18 @@ def test_binning_matches_cpu_stable_sort(which):
     np.testing.assert_array_equal(keys, rk)
     np.testing.assert_array_equal(perm, rp)
     np.testing.assert_array_equal(slot, rs)
+    # the home-block list: every non-empty block once, with its slot range in
+    # the sorted keys; claimed in a spread order, not key order
+    blocks = g.debug_blocks()
+    ublk, first, cnt = np.unique(rk >> 6, return_index=True, return_counts=True)
+    assert blocks.shape == (ublk.size, 4)
+    b = blocks[np.argsort(blocks[:, 0])]
+    np.testing.assert_array_equal(b[:, 0], ublk)
+    np.testing.assert_array_equal(b[:, 1], first)
+    np.testing.assert_array_equal(b[:, 2], first + cnt)
+    np.testing.assert_array_equal(b[:, 3], 0)
+    if ublk.size > 2:
+        assert not np.all(np.diff(blocks[:, 0]) > 0)
     # particles come back in the caller's order after binning
     np.testing.assert_array_equal(g.particles()["x"], sc["x"])
 
