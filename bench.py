@@ -31,6 +31,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "realtime-deformations_amd", "python"))
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (one wave64 vector
+# instruction per 4 clocks per SIMD; FP32 unpacked and FP64 alike)
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
 
 
 def alg_bytes_per_particle(kernel: str, a_over_n: float) -> float:
@@ -184,6 +187,13 @@ def main():
     traffic = _profiled_traffic(dom)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic
+    insts = _profiled_valu(dom)
+    if insts is not None:
+        # the pass is issue-bound: vector instructions (64 lanes each) per
+        # launch, measured by rocprofv3, over this run's mean launch time
+        lane_ops = insts * 64 / (dom_ms / 1e3)
+        out["issue_ceiling"] = {"kernel": dom, "unit": "VALU lane-ops/s", "valu_insts_per_launch": insts,
+                                "achieved": lane_ops, "peak": VALU_PEAK_LANE_OPS, "frac": lane_ops / VALU_PEAK_LANE_OPS}
     out["setup_io"] = setup_and_io(M, sim, sc, dt, host_gen_ms, upload_ms, volumes_ms)
     if args.copy_probe:
         sim.copy_probe(1 << 30)
@@ -416,6 +426,18 @@ def _profiled_traffic(kernel):
         data = json.load(f)
     entry = data.get("kernels", {}).get(kernel)
     return None if entry is None else entry.get("bytes_per_launch")
+
+
+def _profiled_valu(kernel):
+    """Vector instructions per launch of `kernel` (rocprofv3 --pmc
+    SQ_INSTS_VALU, committed summary), or None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "issue.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    entry = data.get("kernels", {}).get(kernel)
+    return None if entry is None else entry.get("SQ_INSTS_VALU")
 
 
 if __name__ == "__main__":

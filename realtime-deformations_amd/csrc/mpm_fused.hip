@@ -183,9 +183,15 @@ __device__ __forceinline__ void g2p_one(const KArgs& a, const Scene& S, const fl
 #pragma unroll
       for (int k = 0; k < 4; ++k) dd[d][k] = (float)(s.ax[d].base - 1 + ((k + r[d]) & 3)) * S.h - xc;
     }
-    const int q0 = s.lb[0] * SX + s.lb[1] * SY + s.lb[2];
+    int qx[4], qy[4], qz[4];  // halo offsets per traversal step and axis
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      qx[k] = (s.lb[0] + ((k + r[0]) & 3)) * SX;
+      qy[k] = (s.lb[1] + ((k + r[1]) & 3)) * SY;
+      qz[k] = s.lb[2] + ((k + r[2]) & 3);
+    }
     g2p_accum(s.ax, dd, [&](int ia, int ib, int ic) {
-      const int q = q0 + ((ia + r[0]) & 3) * SX + ((ib + r[1]) & 3) * SY + ((ic + r[2]) & 3);
+      const int q = qx[ia] + qy[ib] + qz[ic];
       return f3(vh[q], vh[PLANE + q], vh[2 * PLANE + q]);
     }, v, B);
   } else {
@@ -203,7 +209,9 @@ __device__ __forceinline__ void g2p_one(const KArgs& a, const Scene& S, const fl
 }
 
 // P2G scatter of mass and of m v + Q (x_i - x_p) over the 4x4x4 stencil
-// (cpp:99-117 and cpp:249-252 combined); `add(ia, ib, ic, m, px, py, pz)`.
+// (cpp:99-117 and cpp:249-252 combined); `add(ia, ib, ic, ja, jb, jc, m, px,
+// py, pz)` with the traversal steps (ia, ib, ic) and the node offsets
+// (ja, jb, jc) from the stencil base.
 // `s.ax[d].w` hold the weights in traversal order: step k of axis d is node
 // base - 1 + ((k + r[d]) & 3) (r = 0: the plain order).
 template <typename Add>
@@ -228,7 +236,7 @@ __device__ __forceinline__ void p2g_scatter(const Scene& S, const StencilInfo& s
         const int jc = (ic + r[2]) & 3;
         const float ddz = (float)(s.ax[2].base - 1 + jc) * S.h - x.z;
         const float w = wxy * s.ax[2].w[ic];
-        add(ja, jb, jc, m * w, w * fmaf(Q.c[2][0], ddz, qx), w * fmaf(Q.c[2][1], ddz, qy),
+        add(ia, ib, ic, ja, jb, jc, m * w, w * fmaf(Q.c[2][0], ddz, qx), w * fmaf(Q.c[2][1], ddz, qy),
             w * fmaf(Q.c[2][2], ddz, qz));
       }
     }
@@ -246,9 +254,17 @@ __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S,
     lane_rotation(s, r);
 #pragma unroll
     for (int d = 0; d < 3; ++d) rot4(s.ax[d].w, r[d]);
-    const int q0 = s.lb[0] * SX + s.lb[1] * SY + s.lb[2];
-    p2g_scatter(S, s, r, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
-      const int q = q0 + ia * SX + ib * SY + ic;
+    // halo offsets of the rotated traversal, per axis, so a node's address
+    // is one add
+    int qx[4], qy[4], qz[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      qx[k] = (s.lb[0] + ((k + r[0]) & 3)) * SX;
+      qy[k] = (s.lb[1] + ((k + r[1]) & 3)) * SY;
+      qz[k] = s.lb[2] + ((k + r[2]) & 3);
+    }
+    p2g_scatter(S, s, r, x, m, mv, Q, [&](int ia, int ib, int ic, int, int, int, float vm, float px, float py, float pz) {
+      const int q = qx[ia] + qy[ib] + qz[ic];
       atomicAdd(&ah[q], (double)vm);
       atomicAdd(&ah[PLANE + q], (double)px);
       atomicAdd(&ah[2 * PLANE + q], (double)py);
@@ -262,7 +278,7 @@ __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S,
           mark_block(ba, S, (s.ax[0].base - 1 + 3 * di) >> 2, (s.ax[1].base - 1 + 3 * dj) >> 2,
                      (s.ax[2].base - 1 + 3 * dk) >> 2);
     const int r0[3] = {0, 0, 0};
-    p2g_scatter(S, s, r0, x, m, mv, Q, [&](int ia, int ib, int ic, float vm, float px, float py, float pz) {
+    p2g_scatter(S, s, r0, x, m, mv, Q, [&](int, int, int, int ia, int ib, int ic, float vm, float px, float py, float pz) {
       float* g = reinterpret_cast<float*>(
           a.gacc + node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, S));
       atomicAdd(g + 0, vm);
