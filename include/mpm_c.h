@@ -100,7 +100,10 @@ typedef struct mpm_collider {
 void mpm_default_params(mpm_params* p, int32_t ni, int32_t nj, int32_t nk, float h);
 
 /* LagrangeEulerView(max_i, max_j, max_k, particlesNum)
- * (realtime-deformations/material_point_method.cpp:144-154). */
+ * (realtime-deformations/material_point_method.cpp:144-154).
+ * On failure *out is either NULL (invalid arguments) or a context that only
+ * carries the reason (mpm_last_error) and must still go to mpm_destroy; the
+ * same holds for mpm_create_batch and mpm_create_from_snapshot. */
 int mpm_create(const mpm_params* p, int32_t n_particles, mpm_ctx** out);
 int mpm_destroy(mpm_ctx* ctx);
 const char* mpm_last_error(const mpm_ctx* ctx);

@@ -28,7 +28,9 @@ __device__ __forceinline__ int claim_block(int* ctr, int nblocks, int* slot) {
     *slot = s;
   }
   __syncthreads();
-  const int s = *slot;
+  // the same value in every lane: say so, so that the block list entry and
+  // the scene record derived from it are loaded once into scalar registers
+  const int s = __builtin_amdgcn_readfirstlane(*slot);
   return s < nblocks ? s : -1;
 }
 

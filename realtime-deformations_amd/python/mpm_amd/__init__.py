@@ -225,9 +225,14 @@ class Simulation:
         return self, step.value
 
     def _check(self, rc, h=None):
+        """Raise MpmError for a failed call.  h: the handle of a failed create,
+        destroyed here once its message is read."""
         if rc != MPM_OK:
             ctx = h if h is not None else self.ctx
             msg = self.L.mpm_last_error(ctx).decode() if ctx is not None and ctx.value else ""
+            if h is not None and h.value:
+                self.L.mpm_destroy(h)
+                h.value = None
             raise MpmError(rc, msg)
 
     def close(self):
