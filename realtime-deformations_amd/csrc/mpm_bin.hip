@@ -334,17 +334,22 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 // Particle fields and caller index from slot src (current order) to slot
-// dst (binned order); all loads are issued before the first store.
+// dst (binned order), in two halves whose loads are all issued before their
+// stores (half the fields at a time keeps the register count, and with it
+// the number of blocks being planned at once, up).
 __device__ __forceinline__ void move_particle(const BinArgs& b, int src, int dst) {
   const float* __restrict__ P = b.P;
   float* __restrict__ Pout = b.Pout;
-  float v[NFIELDS];
+  constexpr int HALF = NFIELDS / 2;
+#pragma unroll 1
+  for (int f0 = 0; f0 < NFIELDS; f0 += HALF) {
+    float v[HALF];
 #pragma unroll
-  for (int f = 0; f < NFIELDS; ++f) v[f] = P[(int64_t)f * b.stride + src];
-  const int id = b.pid[src];
+    for (int f = 0; f < HALF; ++f) v[f] = P[(int64_t)(f0 + f) * b.stride + src];
 #pragma unroll
-  for (int f = 0; f < NFIELDS; ++f) Pout[(int64_t)f * b.stride + dst] = v[f];
-  b.pid_out[dst] = id;
+    for (int f = 0; f < HALF; ++f) Pout[(int64_t)(f0 + f) * b.stride + dst] = v[f];
+  }
+  b.pid_out[dst] = b.pid[src];
 }
 
 constexpr int RO_T = 256;
