@@ -345,9 +345,12 @@ def bench_adjoint(args):
     rng = np.random.default_rng(9)
     gx = rng.normal(size=(n, 3)).astype(np.float32)
     gv = (rng.normal(size=(n, 3)) * 1e-3).astype(np.float32)
-    # warm-up: one short forward + backward
-    b.adjoint_forward(dt, 20, 20)
-    b.adjoint_backward(gx, gv)
+    # warm-up: each timed configuration once (the library keeps its
+    # checkpoint, record and workspace buffers between calls); the backward
+    # leaves the context at the initial state
+    for every in (20, 1):
+        b.adjoint_forward(dt, steps, every)
+        b.adjoint_backward(gx, gv)
     b.sync()
     t0 = time.perf_counter()
     b.substeps(dt, steps)

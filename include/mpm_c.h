@@ -241,8 +241,12 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms_per_subs
  * gradients).  mpm_adjoint_forward runs nsteps substeps from the current
  * state, keeping a checkpoint every `checkpoint_every` substeps (device
  * memory: 132 bytes per particle per checkpoint; checkpoint_every = 1
- * instead keeps every substep's record, 132 bytes per particle plus 16 bytes
- * per grid node per substep, and the backward re-runs nothing).
+ * instead keeps every substep's record, 132 bytes per particle plus room for
+ * 16 bytes per grid node and 4 per grid block per substep, and the backward
+ * re-runs nothing).  Checkpoints, records and the backward's workspaces stay
+ * allocated for the next call of the same shape, which reuses them; records
+ * are released by a checkpoint_every = 1 call of another length, everything
+ * by mpm_destroy.
  * mpm_adjoint_backward takes
  * dL/dx and dL/dv of the final particles (n x 3, caller order; NULL = 0) and
  * returns dL/dx0, dL/dv0 (n x 3) and dL/dE0 (n) of the state the forward

@@ -437,6 +437,11 @@ int mpm_destroy(mpm_ctx* ctx) {
     void* b[] = {g.st, g.gs, g.gl, g.pid, g.blist, g.nblk};
     for (void* q : b) (void)hipFree(q);
   }
+  {
+    const mpm_ctx::AdjWork& w = ctx->adj_ws;
+    void* b[] = {w.Gc, w.Gs, w.Gs2, w.gEs, w.gEc, w.tmp, w.gnode, w.gmap, w.seg, w.gsave, w.glist};
+    for (void* q : b) (void)hipFree(q);
+  }
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->ev_region)
     if (e) (void)hipEventDestroy(e);
@@ -1064,6 +1069,48 @@ void free_segments(mpm_ctx* ctx) {
 
 // Adjoint kernels over the ns recorded substeps of one segment, last to
 // first; Gs/Gs2 (slot order) swap per substep, Gs holds the result.
+// The reverse sweep's workspaces in ctx->adj_ws, allocated on first use and
+// kept for later calls; the segment buffers grow to `every` substeps.
+int adjoint_workspace(mpm_ctx* ctx, int every) {
+  mpm_ctx::AdjWork& w = ctx->adj_ws;
+  const int64_t sd = ctx->stride;
+  const size_t rows = sizeof(float) * NA * sd;
+  bool ok = true;
+  auto alloc = [&](auto** ptr, size_t bytes) {
+    if (ok && !*ptr && hipMalloc(reinterpret_cast<void**>(ptr), bytes) != hipSuccess) {
+      *ptr = nullptr;
+      ok = false;
+    }
+  };
+  alloc(&w.Gc, rows);
+  alloc(&w.Gs, rows);
+  alloc(&w.Gs2, rows);
+  alloc(&w.gEs, sizeof(float) * sd);
+  alloc(&w.gEc, sizeof(float) * sd);
+  alloc(&w.tmp, sizeof(float) * 3 * (size_t)std::max(1, ctx->n));
+  alloc(&w.gnode, sizeof(float4) * (size_t)ctx->nnodes);
+  alloc(&w.gmap, sizeof(int) * (size_t)ctx->nblocks);
+  if (every > 1 && every > w.seg_every) {
+    // states of a segment's substeps plus the positions after it; the grid
+    // records of each substep
+    void* b[] = {w.seg, w.gsave, w.glist};
+    for (void* q : b) (void)hipFree(q);
+    w.seg = nullptr;
+    w.gsave = nullptr;
+    w.glist = nullptr;
+    w.seg_every = 0;
+    alloc(&w.seg, rows * (size_t)(every + 1));
+    alloc(&w.gsave, sizeof(float4) * (size_t)ctx->nnodes * every);
+    alloc(&w.glist, sizeof(int) * (size_t)(ctx->nblocks + 1) * every);
+    if (ok) w.seg_every = every;
+  }
+  if (!ok) {
+    ctx->err = "device memory for the reverse sweep is exhausted";
+    return MPM_ERR_NOMEM;
+  }
+  return MPM_OK;
+}
+
 // gnode (zero on entry and exit outside the blocks being reversed) holds the
 // node adjoints; map: one int per grid block, -1 on entry and exit.
 void adjoint_segment(mpm_ctx* ctx, float dt, int ns, const float* seg, float4* gs, int* gl, int* map, float*& Gs,
@@ -1092,46 +1139,59 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
   if (check_ctx(ctx) || nsteps < 1 || every < 1) return MPM_ERR_ARG;
   if (int rc = adjoint_supported(ctx)) return rc;
   const int64_t sd = ctx->stride;
-  for (float* c : ctx->adj_ckpt) (void)hipFree(c);
-  ctx->adj_ckpt.clear();
-  free_segments(ctx);
   ctx->adj_steps = 0;
+  // checkpoint buffers and per-substep records are kept from the previous
+  // call and reused when the shape matches
+  const int L = std::max(1, ctx->resort_every);
+  std::vector<int> seg_ns;
+  if (every == 1)
+    for (int done = 0; done < nsteps; done += L) seg_ns.push_back(std::min(L, nsteps - done));
+  bool same = seg_ns.size() == ctx->adj_segs.size();
+  for (size_t i = 0; same && i < seg_ns.size(); ++i) same = ctx->adj_segs[i].ns == seg_ns[i];
+  if (every == 1 && !same) free_segments(ctx);  // other calls leave them for the next keep-all call
   if (!ctx->adj_const) MPM_HIP(hipMalloc(&ctx->adj_const, sizeof(float) * 3 * sd));
   launch_rows_to_caller(ctx->adj_const, ctx->P + (int64_t)FM * sd, ctx->pid, sd, ctx->n, 3, ctx->stream);
   const int nck = every == 1 ? 1 : (nsteps + every - 1) / every;
+  while ((int)ctx->adj_ckpt.size() > nck) {
+    (void)hipFree(ctx->adj_ckpt.back());
+    ctx->adj_ckpt.pop_back();
+  }
   for (int c = 0; c < nck; ++c) {
-    float* buf = nullptr;
-    MPM_HIP(hipMalloc(&buf, sizeof(float) * NA * sd));
-    ctx->adj_ckpt.push_back(buf);
-    launch_rows_to_caller(buf, ctx->P, ctx->pid, sd, ctx->n, NA, ctx->stream);
+    if (c == (int)ctx->adj_ckpt.size()) {
+      float* buf = nullptr;
+      MPM_HIP(hipMalloc(&buf, sizeof(float) * NA * sd));
+      ctx->adj_ckpt.push_back(buf);
+    }
+    launch_rows_to_caller(ctx->adj_ckpt[c], ctx->P, ctx->pid, sd, ctx->n, NA, ctx->stream);
     if (every > 1)
       if (int rc = mpm_substeps(ctx, dt, std::min(every, nsteps - c * every))) return rc;
   }
   if (every == 1) {
     // keep every substep: segments of resort_every substeps, one binning each
-    const int L = std::max(1, ctx->resort_every);
     const size_t rows = sizeof(float) * NA * sd;
-    for (int done = 0; done < nsteps;) {
-      mpm_ctx::AdjSegment g;
-      g.ns = std::min(L, nsteps - done);
-      bool ok = hipMalloc(&g.st, rows * (size_t)(g.ns + 1)) == hipSuccess &&
-                hipMalloc(&g.gs, sizeof(float4) * (size_t)ctx->nnodes * g.ns) == hipSuccess &&
-                hipMalloc(&g.gl, sizeof(int) * (size_t)(ctx->nblocks + 1) * g.ns) == hipSuccess &&
-                hipMalloc(&g.pid, sizeof(int) * (size_t)std::max(1, ctx->n)) == hipSuccess &&
-                hipMalloc(&g.blist, sizeof(int4) * (size_t)ctx->nblocks) == hipSuccess &&
-                hipMalloc(&g.nblk, sizeof(int)) == hipSuccess;
-      ctx->adj_segs.push_back(g);
-      if (!ok) {
-        free_segments(ctx);
-        ctx->err = "device memory for the per-substep records (checkpoint_every = 1) is exhausted";
-        return MPM_ERR_NOMEM;
+    for (size_t i = 0; i < seg_ns.size(); ++i) {
+      if (i == ctx->adj_segs.size()) {
+        mpm_ctx::AdjSegment g;
+        g.ns = seg_ns[i];
+        bool ok = hipMalloc(&g.st, rows * (size_t)(g.ns + 1)) == hipSuccess &&
+                  hipMalloc(&g.gs, sizeof(float4) * (size_t)ctx->nnodes * g.ns) == hipSuccess &&
+                  hipMalloc(&g.gl, sizeof(int) * (size_t)(ctx->nblocks + 1) * g.ns) == hipSuccess &&
+                  hipMalloc(&g.pid, sizeof(int) * (size_t)std::max(1, ctx->n)) == hipSuccess &&
+                  hipMalloc(&g.blist, sizeof(int4) * (size_t)ctx->nblocks) == hipSuccess &&
+                  hipMalloc(&g.nblk, sizeof(int)) == hipSuccess;
+        ctx->adj_segs.push_back(g);
+        if (!ok) {
+          free_segments(ctx);
+          ctx->err = "device memory for the per-substep records (checkpoint_every = 1) is exhausted";
+          return MPM_ERR_NOMEM;
+        }
       }
+      const mpm_ctx::AdjSegment& g = ctx->adj_segs[i];
       if (int rc = rebin(ctx)) return rc;
       MPM_HIP(hipMemcpyAsync(g.pid, ctx->pid, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
       MPM_HIP(hipMemcpyAsync(g.blist, ctx->blist, sizeof(int4) * ctx->nblocks, hipMemcpyDeviceToDevice, ctx->stream));
       MPM_HIP(hipMemcpyAsync(g.nblk, ctx->nblk, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
       if (int rc = record_segment(ctx, dt, g.ns, g.st, g.gs, g.gl)) return rc;
-      done += g.ns;
     }
     ctx->gacc_clean = false;
   }
@@ -1153,39 +1213,15 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   const float dt = ctx->adj_dt;
   const int every = ctx->adj_every, steps = ctx->adj_steps;
   hipStream_t s = ctx->stream;
-  float *Gc = nullptr, *Gs = nullptr, *Gs2 = nullptr, *gEs = nullptr, *gEc = nullptr, *seg = nullptr, *tmp = nullptr;
-  float4 *gnode = nullptr, *gsave = nullptr;
-  int *glist = nullptr, *gmap = nullptr;
   const size_t rows = sizeof(float) * NA * sd;
-  bool ok = true;
-  auto alloc = [&](auto** ptr, size_t bytes) {
-    if (ok && hipMalloc(reinterpret_cast<void**>(ptr), bytes) != hipSuccess) ok = false;
-  };
-  alloc(&Gc, rows);
-  alloc(&Gs, rows);
-  alloc(&Gs2, rows);
-  alloc(&gEs, sizeof(float) * sd);
-  alloc(&gEc, sizeof(float) * sd);
-  // states of a segment's substeps plus the positions after it; the P2G
-  // sums of each substep
-  if (every > 1) {
-    alloc(&seg, rows * (size_t)(every + 1));
-    alloc(&gsave, sizeof(float4) * (size_t)ctx->nnodes * every);
-    alloc(&glist, sizeof(int) * (size_t)(ctx->nblocks + 1) * every);
-  }
-  alloc(&gmap, sizeof(int) * (size_t)ctx->nblocks);
-  alloc(&tmp, sizeof(float) * 3 * (size_t)std::max(1, n));
-  alloc(&gnode, sizeof(float4) * (size_t)ctx->nnodes);
+  if (int rc = adjoint_workspace(ctx, every)) return rc;
+  mpm_ctx::AdjWork& w = ctx->adj_ws;
+  float *const Gc = w.Gc, *const gEs = w.gEs, *const gEc = w.gEc, *const seg = w.seg, *const tmp = w.tmp;
+  float*& Gs = w.Gs;  // Gs and Gs2 swap per substep
+  float*& Gs2 = w.Gs2;
+  float4 *const gnode = w.gnode, *const gsave = w.gsave;
+  int *const glist = w.glist, *const gmap = w.gmap;
   int rc = MPM_OK;
-  auto cleanup = [&] {
-    void* b[] = {Gc, Gs, Gs2, gEs, gEc, seg, tmp, gnode, gsave, glist, gmap};
-    for (void* p : b) (void)hipFree(p);
-  };
-  if (!ok) {
-    cleanup();
-    ctx->err = "device allocation failed";
-    return MPM_ERR_NOMEM;
-  }
   MPM_HIP(hipMemsetAsync(Gc, 0, rows, s));
   MPM_HIP(hipMemsetAsync(gEc, 0, sizeof(float) * sd, s));
   launch_clear(gnode, ctx->nnodes, s);
@@ -1249,7 +1285,6 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
     MPM_HIP(hipStreamSynchronize(s));
     rc = read_status(ctx);
   }
-  cleanup();
   return rc;
 }
 

@@ -161,7 +161,17 @@ struct mpm_ctx {
     int4* blist = nullptr;   // nblocks
     int* nblk = nullptr;     // 1
   };
-  std::vector<AdjSegment> adj_segs;
+  std::vector<AdjSegment> adj_segs;  // kept between forward calls of the same shape
+  // reverse-sweep workspaces, kept between calls (freed by mpm_destroy)
+  struct AdjWork {
+    float *Gc = nullptr, *Gs = nullptr, *Gs2 = nullptr, *gEs = nullptr, *gEc = nullptr, *tmp = nullptr;
+    float4* gnode = nullptr;
+    int* gmap = nullptr;
+    float* seg = nullptr;     // (every + 1) x 33 rows
+    float4* gsave = nullptr;  // every x nnodes
+    int* glist = nullptr;     // every x (nblocks + 1)
+    int seg_every = 0;        // substeps seg / gsave / glist hold
+  } adj_ws;
   // positions readback: device staging, pinned host mirror, copy stream
   float* pos_stage = nullptr;
   float* pos_host = nullptr;     // pinned
