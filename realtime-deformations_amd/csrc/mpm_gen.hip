@@ -251,6 +251,67 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_single(const int* in, int* out,
   if (t == 0) *total = carry;
 }
 
+// Exclusive prefix sum of n ints (in place allowed) over many workgroups:
+// the sums of tiles of PS_T * PS_ITEMS ints, k_scan_single over those sums,
+// then each tile's local scan offset by its sum.  *total = the sum.
+constexpr int PS_T = 256, PS_ITEMS = 8, PS_TILE = PS_T * PS_ITEMS;
+inline int scan_tiles(int n) { return (n + PS_TILE - 1) / PS_TILE; }
+__global__ __launch_bounds__(PS_T) void k_scan_tile_sums(const int* in, int n, int* wsum) {
+  __shared__ int ws[PS_T / 64];
+  const int base = blockIdx.x * PS_TILE + threadIdx.x * PS_ITEMS;
+  int s = 0;
+#pragma unroll
+  for (int u = 0; u < PS_ITEMS; ++u) s += (base + u < n) ? in[base + u] : 0;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < PS_T / 64; ++w) t += ws[w];
+    wsum[blockIdx.x] = t;
+  }
+}
+__global__ __launch_bounds__(PS_T) void k_scan_tiles(const int* in, int* out, int n, const int* woff) {
+  __shared__ int ws[PS_T / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int base = blockIdx.x * PS_TILE + threadIdx.x * PS_ITEMS;
+  int v[PS_ITEMS], s = 0;
+#pragma unroll
+  for (int u = 0; u < PS_ITEMS; ++u) {
+    v[u] = (base + u < n) ? in[base + u] : 0;
+    s += v[u];
+  }
+  int incl = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) ws[wv] = incl;
+  __syncthreads();
+  int run = woff[blockIdx.x] + incl - s;
+  for (int w = 0; w < wv; ++w) run += ws[w];
+#pragma unroll
+  for (int u = 0; u < PS_ITEMS; ++u)
+    if (base + u < n) {
+      out[base + u] = run;
+      run += v[u];
+    }
+}
+// wsum: scan_tiles(n) ints of scratch
+void launch_scan(const int* in, int* out, int n, int* total, int* wsum, hipStream_t st) {
+  const int nt = scan_tiles(n);
+  if (nt == 0) {
+    k_scan_single<<<1, SCAN_T, 0, st>>>(in, out, 0, total);
+    return;
+  }
+  k_scan_tile_sums<<<nt, PS_T, 0, st>>>(in, n, wsum);
+  k_scan_single<<<1, SCAN_T, 0, st>>>(wsum, wsum, nt, total);
+  k_scan_tiles<<<nt, PS_T, 0, st>>>(in, out, n, wsum);
+}
+
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 uint64_t gcd64(uint64_t a, uint64_t b) {
@@ -345,20 +406,20 @@ int mpm_scene_generate(mpm_ctx* ctx, int32_t scene, const mpm_generator* g, int6
       GEN_FAIL(MPM_ERR_ARG, "CELLS: the scene's particle count is outside [ppc_lo, ppc_hi] x cells");
     const int nc = (int)nc64;
     const uint64_t A = gen_cell_order_multiplier((uint64_t)nc), B = g->seed % (uint64_t)nc;
-    GEN_HIP(hipMalloc(&buf, sizeof(int) * 3 * (size_t)nc));
-    int *cnt = buf, *off = buf + nc, *flag = buf + 2 * nc;
+    GEN_HIP(hipMalloc(&buf, sizeof(int) * (3 * (size_t)nc + scan_tiles(nc))));
+    int *cnt = buf, *off = buf + nc, *flag = buf + 2 * nc, *wsum = buf + 3 * (size_t)nc;
     k_gen_cell_counts<<<grid_for(nc), 256, 0, ctx->stream>>>(a, nc, cnt);
     // +-1 passes in the seeded cell order until the total is exactly n (one
     // pass unless |total - n| exceeds the cells that can still move)
     for (int pass = 0;; ++pass) {
       int total = 0;
-      k_scan_single<<<1, SCAN_T, 0, ctx->stream>>>(cnt, off, nc, tot);
+      launch_scan(cnt, off, nc, tot, wsum, ctx->stream);
       GEN_HIP(hipMemcpyAsync(&total, tot, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
       GEN_HIP(hipStreamSynchronize(ctx->stream));
       if (total == S.np) break;
       if (pass > g->ppc_hi - g->ppc_lo) GEN_FAIL(MPM_ERR_HIP, "CELLS: count adjustment did not converge");
       k_gen_adjust_flags<<<grid_for(nc), 256, 0, ctx->stream>>>(a, nc, A, B, cnt, tot, flag);
-      k_scan_single<<<1, SCAN_T, 0, ctx->stream>>>(flag, off, nc, tot + 1);
+      launch_scan(flag, off, nc, tot + 1, wsum, ctx->stream);
       k_gen_adjust_apply<<<grid_for(nc), 256, 0, ctx->stream>>>(a, nc, A, B, cnt, flag, off, tot);
     }
     if (S.np) k_gen_cells<<<grid_for(S.np), 256, 0, ctx->stream>>>(a, nc, off);
@@ -374,10 +435,10 @@ int mpm_scene_generate(mpm_ctx* ctx, int32_t scene, const mpm_generator* g, int6
     const int ncand = (int)nq;
     int kept = 0;
     if (ncand) {
-      GEN_HIP(hipMalloc(&buf, sizeof(int) * 2 * (size_t)ncand));
-      int *flag = buf, *rank = buf + ncand;
+      GEN_HIP(hipMalloc(&buf, sizeof(int) * (2 * (size_t)ncand + scan_tiles(ncand))));
+      int *flag = buf, *rank = buf + ncand, *wsum = buf + 2 * (size_t)ncand;
       k_gen_lattice_flags<<<grid_for(ncand), 256, 0, ctx->stream>>>(a, lt, ncand, flag);
-      k_scan_single<<<1, SCAN_T, 0, ctx->stream>>>(flag, rank, ncand, tot);
+      launch_scan(flag, rank, ncand, tot, wsum, ctx->stream);
       GEN_HIP(hipMemcpyAsync(&kept, tot, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
       GEN_HIP(hipStreamSynchronize(ctx->stream));
     }
