@@ -261,11 +261,15 @@ __global__ void __launch_bounds__(BS_T) k_block_scan(uint32_t* wgcnt, int nwg, i
 }
 
 // The block passes claim home blocks in list order.  Rank r in key order
-// goes to list slot (r * CLAIM_STRIDE) mod nblk, so that blocks claimed at
-// the same time lie apart and do not flush into the same grid nodes at once.
-constexpr uint32_t CLAIM_STRIDE = 7919, CLAIM_STRIDE_ALT = 7907;  // primes
+// goes to list slot (r * stride) mod nblk, so that blocks claimed at the same
+// time lie apart and do not flush into the same grid nodes at once.
+// The stride is the first of three primes that does not divide nblk; nblk
+// < 2^31 cannot be a multiple of all three (their product exceeds 2^38), so
+// the map is always a permutation.
+constexpr uint32_t CLAIM_STRIDES[3] = {7919, 7907, 7901};
 __device__ __forceinline__ uint32_t claim_slot(uint32_t r, uint32_t nblk) {
-  const uint64_t m = nblk % CLAIM_STRIDE ? CLAIM_STRIDE : CLAIM_STRIDE_ALT;  // coprime with nblk
+  const uint64_t m = nblk % CLAIM_STRIDES[0] ? CLAIM_STRIDES[0]
+                     : nblk % CLAIM_STRIDES[1] ? CLAIM_STRIDES[1] : CLAIM_STRIDES[2];
   return (uint32_t)(((uint64_t)r * m) % nblk);
 }
 
