@@ -143,6 +143,48 @@ def test_keep_every_substep_matches_checkpoints(case):
     g.close()
 
 
+def _scene_b():
+    sc = scenes.snowball(seed=22, n=1500, grid=40, radius=0.1, centre=(0.5, 0.55, 0.5), v0=(0.5, -0.5, 0.0), E=1.5e5)
+    c = np.float32([0.5, 0.55, 0.5])
+    sc["v"] = (sc["v"] - np.float32(4.0) * (sc["x"] - c)).astype(np.float32)
+    return sc
+
+
+def test_batched_adjoint_equals_each_scene_alone(case):
+    """Two scenes (other grid, spacing, modulus and motion) reversed in one
+    batched context: each scene's gradients equal its own single-scene
+    reverse sweep.  The batch bins and sums in another order, so the
+    tolerances are those measured across checkpoint intervals: 1e-5 for
+    dL/dx0 and dL/dv0, 1e-3 for dL/dE0."""
+    sa, sb = case["sc"], _scene_b()
+    na, nb = sa["x"].shape[0], sb["x"].shape[0]
+    rng = np.random.default_rng(6)
+    ab = rng.normal(size=(nb, 3)).astype(np.float32)
+    bb = rng.normal(size=(nb, 3)).astype(np.float32) * np.float32(1e-3)
+    gb = _gpu(sb)
+    gb.set_particles(sb["x"], sb["v"], sb["m"], sb["E0"])
+    gb.init_volumes()
+    gb.adjoint_forward(DT, STEPS, EVERY)
+    want_b = gb.adjoint_backward(ab, bb)
+    gb.close()
+
+    def params(sc):
+        return M.default_params(sc["dims"], sc["h"], compose=M.COMPOSE_TEXTBOOK, bc_mode=M.BC_DOMAIN, theta_s=7.5e-3)
+
+    batch = M.Simulation([params(sa), params(sb)], [na, nb])
+    batch.scene_set_particles(0, sa["x"], sa["v"], sa["m"], sa["E0"])
+    batch.scene_set_particles(1, sb["x"], sb["v"], sb["m"], sb["E0"])
+    batch.init_volumes()
+    batch.adjoint_forward(DT, STEPS, EVERY)
+    got = batch.adjoint_backward(np.concatenate([case["a"], ab]), np.concatenate([case["b"], bb]))
+    batch.close()
+    want_a = (case["gx0"], case["gv0"], case["gE0"])
+    for i, (ref_a, ref_b, tol) in enumerate(zip(want_a, want_b, (1e-5, 1e-5, 1e-3))):
+        for part, ref in ((got[i][:na], ref_a), (got[i][na:], ref_b)):
+            err = np.linalg.norm(part.astype(np.float64) - ref) / np.linalg.norm(ref.astype(np.float64))
+            assert err < tol, (i, err, tol)
+
+
 def test_torch_autograd_function_matches_adjoint(case):
     torch = pytest.importorskip("torch")
     from mpm_amd.autograd import Rollout
