@@ -112,6 +112,11 @@ def test_batch_scenes_generated_independently():
     b.init_volumes()
     b.substeps(1e-4, 3)
     assert b.scene_status() == [0, 0, 0]
+    # positions readback of a batch: the scenes concatenated, each in caller order
+    want = np.concatenate([b.scene_particles(s)["x"] for s in range(3)])
+    np.testing.assert_array_equal(b.positions(), want)
+    b.positions_begin()
+    np.testing.assert_array_equal(b.positions_wait(), want)
     b.close()
 
 
