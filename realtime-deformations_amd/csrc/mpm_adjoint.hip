@@ -11,8 +11,10 @@
 //   G2P   v' = sum_i w v_i,  B' = sum_i w v_i (X_i - x)^T
 //   x'    = clamp(x + dt v')
 // The reverse runs these backwards from the stored state of substep n:
+//   k_adj_map       the block map of the substep's grid record (set before,
+//                   reset with the node adjoints cleared after)
 //   k_adj_g2p_block advection and G2P (mpm_fused.hip, on the block halos):
-//                   node velocities from the substep's saved P2G sums, node
+//                   node velocities from the substep's recorded P2G sums, node
 //                   velocity adjoints and the position adjoint through the
 //                   weights
 //   k_adj_grid      wall mask and v = P/m + dt g: adjoints of (m_i, P_i),

@@ -5,11 +5,12 @@
 // 4x4x4-cell blocks numbered blk0 + (bi * nb1 + bj) * nb2 + bk, blk0 being the
 // first block of the particle's scene: one sort bins every scene.  A stable LSD radix
 // sort (8-bit digits) orders (key, index) pairs exactly like a CPU stable
-// sort: ties keep index order.  Each non-empty block's particles are then
-// moved into one contiguous range, laid out round-robin over the block's 64
-// cells (slot s holds the r-th particle of the k-th cell that has more than r
-// particles), so the 64 lanes of a wave usually touch 64 different stencil
-// origins and the LDS halo atomics of P2G do not collide on one address.
+// sort: ties keep index order.  The non-empty blocks are listed from
+// per-workgroup counts and a scan (no atomics), in a spread claim order
+// (claim_slot).  Each block's particles are then moved into one contiguous
+// range in column groups (see the layout note above k_reorder), so the lanes
+// of each 16-lane group start their stencils in different (y, z) columns and
+// the FP64 LDS atomics of P2G spread over the bank pairs.
 #include "mpm_bin.hpp"
 
 namespace mpm {
