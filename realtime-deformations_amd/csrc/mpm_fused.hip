@@ -12,15 +12,16 @@
 //   m v + (m B D^-1 - dt V0 D^-1 P F^T)(x_i - x_p), into grid n (cpp:94-129)
 //
 // so v and B never leave registers inside a run of substeps.  One workgroup
-// processes one 4x4x4-cell home block at a time (persistent loop over the
-// block list built by the binning pass): the grid velocities around the block
-// and the accumulators it writes live in LDS halos of 9x9x9 nodes (the block,
-// the cubic stencil, and a one-cell margin for particles that drifted since
-// the last sort).  Halo strides (112, 12, 1) make a 4x4x4 cube of lanes —
-// what the round-robin particle layout produces — hit 64 distinct banks for
-// any offset.  A particle outside the margin takes a global-memory path.
-// The grid pass then turns accumulators into velocities on the active
-// blocks only.
+// processes one 4x4x4-cell home block at a time, claiming blocks from the
+// binning pass's list through a device counter: the grid velocities around
+// the block and the FP64 accumulators it writes live in LDS halos of 9x9x9
+// nodes (the block, the cubic stencil, and a one-cell margin for particles
+// that drifted since the last sort).  With halo strides (112, 12, 1) an FP64
+// atomic's bank pair depends on the node's (y, z) column; the binning's
+// column-group layout and a per-lane rotation of the stencil traversal
+// (lane_rotation) spread each 16-lane group over distinct pairs.  A particle
+// outside the margin takes a global-memory path.  The grid pass then turns
+// accumulators into velocities on the active blocks only.
 //
 // Every block-list entry carries its scene: one launch steps all scenes of a
 // context, each with its own grid spacing, grid, material and colliders read
