@@ -346,10 +346,10 @@ __global__ void __launch_bounds__(TPB) k_rows_to_slots(float* dst, const float* 
   if (p >= n) return;
   copy_rows(dst, p, src, pid[p], stride, rows);
 }
-__global__ void __launch_bounds__(TPB) k_rows_to_caller(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows) {
+// inv = pid^-1: the slot of each caller index
+__global__ void __launch_bounds__(TPB) k_invert(const int* pid, int* inv, int n) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  copy_rows(dst, pid[p], src, p, stride, rows);
+  if (p < n) inv[pid[p]] = p;
 }
 __global__ void __launch_bounds__(TPB) k_add_to_caller(float* dst, const float* src, const int* pid, int n) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -386,9 +386,13 @@ void launch_rows_to_slots(float* dst, const float* src, const int* pid, int64_t 
                           hipStream_t s) {
   if (n) k_rows_to_slots<<<nblk(n), TPB, 0, s>>>(dst, src, pid, stride, n, rows);
 }
-void launch_rows_to_caller(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows,
+void launch_rows_to_caller(float* dst, const float* src, const int* pid, int* inv, int64_t stride, int n, int rows,
                            hipStream_t s) {
-  if (n) k_rows_to_caller<<<nblk(n), TPB, 0, s>>>(dst, src, pid, stride, n, rows);
+  // written as a gather through the inverse permutation: coalesced stores,
+  // where scattering the rows to caller order would store 4 bytes per line
+  if (!n) return;
+  k_invert<<<nblk(n), TPB, 0, s>>>(pid, inv, n);
+  k_rows_to_slots<<<nblk(n), TPB, 0, s>>>(dst, src, inv, stride, n, rows);
 }
 void launch_add_to_caller(float* dst, const float* src, const int* pid, int n, hipStream_t s) {
   if (n) k_add_to_caller<<<nblk(n), TPB, 0, s>>>(dst, src, pid, n);

@@ -17,8 +17,8 @@ void launch_adj_p2g(const KArgs& a, const float* St, const float* Gin, float* Go
 // row transfers: slot order (pid: caller index per slot) <-> caller order
 void launch_rows_to_slots(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows,
                           hipStream_t s);
-void launch_rows_to_caller(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows,
-                           hipStream_t s);
+void launch_rows_to_caller(float* dst, const float* src, const int* pid, int* inv, int64_t stride, int n, int rows,
+                           hipStream_t s);  // inv: n ints of scratch
 void launch_add_to_caller(float* dst, const float* src, const int* pid, int n, hipStream_t s);
 // interleaved host layout (n x comps) <-> rows
 void launch_aos_to_rows(float* dst, const float* src, int n, int comps, int64_t stride, hipStream_t s);

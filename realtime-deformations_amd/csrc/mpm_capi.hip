@@ -433,6 +433,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   for (DevCollider* c : ctx->scoll) (void)hipFree(c);
   for (float* c : ctx->adj_ckpt) (void)hipFree(c);
   (void)hipFree(ctx->adj_const);
+  (void)hipFree(ctx->adj_inv);
   for (auto& g : ctx->adj_segs) {
     void* b[] = {g.st, g.gs, g.gl, g.pid, g.blist, g.nblk};
     for (void* q : b) (void)hipFree(q);
@@ -1155,7 +1156,8 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
   for (size_t i = 0; same && i < seg_ns.size(); ++i) same = ctx->adj_segs[i].ns == seg_ns[i];
   if (every == 1 && !same) free_segments(ctx);  // other calls leave them for the next keep-all call
   if (!ctx->adj_const) MPM_HIP(hipMalloc(&ctx->adj_const, sizeof(float) * 3 * sd));
-  launch_rows_to_caller(ctx->adj_const, ctx->P + (int64_t)FM * sd, ctx->pid, sd, ctx->n, 3, ctx->stream);
+  if (!ctx->adj_inv) MPM_HIP(hipMalloc(&ctx->adj_inv, sizeof(int) * (size_t)std::max(1, ctx->n)));
+  launch_rows_to_caller(ctx->adj_const, ctx->P + (int64_t)FM * sd, ctx->pid, ctx->adj_inv, sd, ctx->n, 3, ctx->stream);
   const int nck = every == 1 ? 1 : (nsteps + every - 1) / every;
   while ((int)ctx->adj_ckpt.size() > nck) {
     (void)hipFree(ctx->adj_ckpt.back());
@@ -1167,7 +1169,7 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
       MPM_HIP(hipMalloc(&buf, sizeof(float) * NA * sd));
       ctx->adj_ckpt.push_back(buf);
     }
-    launch_rows_to_caller(ctx->adj_ckpt[c], ctx->P, ctx->pid, sd, ctx->n, NA, ctx->stream);
+    launch_rows_to_caller(ctx->adj_ckpt[c], ctx->P, ctx->pid, ctx->adj_inv, sd, ctx->n, NA, ctx->stream);
     if (every > 1)
       if (int rc = mpm_substeps(ctx, dt, std::min(every, nsteps - c * every))) return rc;
   }
@@ -1254,7 +1256,7 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
       launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
       MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
       adjoint_segment(ctx, dt, g.ns, g.st, g.gs, g.gl, gmap, Gs, Gs2, gnode, gEs);
-      launch_rows_to_caller(Gc, Gs, ctx->pid, sd, n, NA, s);
+      launch_rows_to_caller(Gc, Gs, ctx->pid, ctx->adj_inv, sd, n, NA, s);
       launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
     }
   }
@@ -1267,7 +1269,7 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
     launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
     MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
     adjoint_segment(ctx, dt, ns, seg, gsave, glist, gmap, Gs, Gs2, gnode, gEs);
-    launch_rows_to_caller(Gc, Gs, ctx->pid, sd, n, NA, s);
+    launch_rows_to_caller(Gc, Gs, ctx->pid, ctx->adj_inv, sd, n, NA, s);
     launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
     ctx->gacc_clean = false;
   }

@@ -149,6 +149,7 @@ struct mpm_ctx {
   int adj_steps = 0, adj_every = 0;
   std::vector<float*> adj_ckpt;
   float* adj_const = nullptr;
+  int* adj_inv = nullptr;  // slot of each caller index (row transfers to caller order)
   // checkpoint_every == 1: every substep's record kept, segment by segment
   // (one binning each): starting states (+ the positions after the segment),
   // P2G sums, and the segment's slot order and block list
