@@ -120,6 +120,26 @@ def test_snowball_fused_1_and_100_substeps():
     _compare_particles(g.particles(), o.particles(), 1e-3, h=sc["h"])
 
 
+def test_snowball_free_fall_closed_form_on_the_device():
+    """The 8,192-particle snowball never touches a boundary in 100 substeps:
+    uniform v, B = 0, F = I, so the fused path itself must give
+    x_n = x_0 + n dt v0 + dt^2 g n(n+1)/2 and v_n = v0 + n dt g (the
+    mathematics, independent of the oracle; FP32 tolerances as the oracle's
+    own check)."""
+    sc = scenes.snowball()
+    g, _ = _pair(sc)
+    g.init_volumes()
+    n, dt = 100, sc["dt"]
+    g.substeps(dt, n)
+    P = g.particles()
+    grav = np.array([0.0, -9.8, 0.0])
+    expect = sc["x"].astype(np.float64) + n * dt * sc["v"] + dt * dt * grav * n * (n + 1) / 2
+    assert np.abs(P["x"] - expect).max() < 2e-6
+    np.testing.assert_allclose(P["v"], sc["v"] + n * dt * grav, atol=2e-5)
+    np.testing.assert_allclose(P["FE"], eye9(P["FE"].shape[0]), atol=1e-5)
+    g.close()
+
+
 def test_warm_state_one_substep():
     """A warm state (non-zero B, F_E off identity, F_P != I) and one fused
     substep: exercises stress, hardening and the clamp."""
