@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -242,7 +243,23 @@ int mpm_get_positions(mpm_ctx* ctx, float* x) {
   const float* view = nullptr;
   int rc;
   if ((rc = mpm_positions_begin(ctx)) || (rc = mpm_positions_wait(ctx, &view))) return rc;
-  std::memcpy(x, view, sizeof(float) * 3 * (size_t)ctx->n);
+  // from the pinned mirror into the caller's array: one host thread copies
+  // (and first-touches) about 9 GB/s, so large reads are split over threads
+  const size_t bytes = sizeof(float) * 3 * (size_t)ctx->n;
+  const size_t chunk = size_t(4) << 20;
+  const unsigned nt = (unsigned)std::min<size_t>(8, std::max<size_t>(1, bytes / chunk));
+  if (nt == 1) {
+    std::memcpy(x, view, bytes);
+  } else {
+    const size_t part = (bytes / nt + 63) / 64 * 64;
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) {
+      const size_t o = std::min(bytes, part * t), e = std::min(bytes, part * (t + 1));
+      pool.emplace_back([=] { std::memcpy(reinterpret_cast<char*>(x) + o, reinterpret_cast<const char*>(view) + o, e - o); });
+    }
+    std::memcpy(x, view, std::min(bytes, part));
+    for (std::thread& th : pool) th.join();
+  }
   return mpm_sync(ctx);
 }
 
