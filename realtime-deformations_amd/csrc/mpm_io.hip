@@ -252,12 +252,20 @@ int mpm_get_positions(mpm_ctx* ctx, float* x) {
     std::memcpy(x, view, bytes);
   } else {
     const size_t part = (bytes / nt + 63) / 64 * 64;
-    std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nt; ++t) {
+    auto copy_part = [=](unsigned t) {
       const size_t o = std::min(bytes, part * t), e = std::min(bytes, part * (t + 1));
-      pool.emplace_back([=] { std::memcpy(reinterpret_cast<char*>(x) + o, reinterpret_cast<const char*>(view) + o, e - o); });
+      std::memcpy(reinterpret_cast<char*>(x) + o, reinterpret_cast<const char*>(view) + o, e - o);
+    };
+    std::vector<std::thread> pool;
+    unsigned handed = 1;
+    try {
+      pool.reserve(nt - 1);
+      for (; handed < nt; ++handed) pool.emplace_back(copy_part, handed);
+    } catch (...) {
+      // fewer threads than asked for: the parts not handed out are copied here
     }
-    std::memcpy(x, view, std::min(bytes, part));
+    copy_part(0);
+    for (unsigned t = handed; t < nt; ++t) copy_part(t);
     for (std::thread& th : pool) th.join();
   }
   return mpm_sync(ctx);
