@@ -1,10 +1,12 @@
 """Host-side checks that need no GPU: the C ABI library loads and exports
 every function include/mpm_c.h declares; the scene generators meet their
 specifications; the CPU binning reference is a valid stable sort with the
-documented in-block slot layout."""
+documented in-block slot layout; bench.py refuses more than one GPU."""
 import ctypes
 import os
 import re
+import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -29,6 +31,13 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert sorted(M.EXPORTS) == sorted(names)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
 
 
 def test_default_params_are_the_reference_constants():
