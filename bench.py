@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the MPM snow substep on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-autotune]
 
 Workload: the 4,194,304-particle snow slab on a 256^3 grid (dx = 1/256,
 4x4x4 node blocks), per-particle E0 in [1e5, 2e5], dt = 2.5e-5 — the
@@ -10,7 +10,11 @@ all particles (binning, stress + F update + P2G, grid update + boundary
 conditions, G2P + advection), inputs resident in HBM.  Prints one JSON line.
 
 value: K substeps replayed as the library runs them (HIP-graph segments of
-re-bin + 16 substeps), wall clock between stream synchronisations.
+one re-bin + resort_every substeps), wall clock between stream
+synchronisations.  Before timing, the library's autotuner picks the re-bin
+interval and block-pass width (reported under "autotune"; results do not
+depend on either); --no-autotune times the defaults (16 substeps, 2,048
+workgroups).
 roofline: the dominant kernel's algorithmic bytes per launch (per-particle
 bytes x particles, see alg_bytes_per_particle) over its mean launch time,
 measured with HIP events on the library's stream around every launch of a
@@ -101,7 +105,10 @@ def main():
                     help="slab4m: the headline 4M-particle slab; batch64: 64 independent scenes in one "
                          "context against the same scenes run one after another; adjoint16: forward and "
                          "reverse mode over 16 scenes of 32,768 particles")
-    ap.add_argument("--autotune", action="store_true", help="batch64: autotune the batch before timing")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch64: run the library's autotuner (re-bin interval, block-pass width) before timing")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="slab4m: time the library's default tuning instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); "
@@ -133,6 +140,12 @@ def main():
     volumes_ms = (time.perf_counter() - t0) * 1e3
     V0 = sim.particles()["V0"] if not args.no_cpu_baseline else None
     dt = sc["dt"]
+    tuned = None
+    if not args.no_autotune:
+        # mpm_autotune times each candidate from the current state and
+        # restores it; the timed substeps below start from the same state
+        tuned = dict(ms_per_substep=sim.autotune(dt, 64))
+        tuned["resort_every"], tuned["block_grid"] = sim.tuning()
 
     sim.substeps(dt, args.warmup)
     sim.sync()
@@ -195,6 +208,8 @@ def main():
         lane_ops = insts * 64 / (dom_ms / 1e3)
         out["issue_ceiling"] = {"kernel": dom, "unit": "VALU lane-ops/s", "valu_insts_per_launch": insts,
                                 "achieved": lane_ops, "peak": VALU_PEAK_LANE_OPS, "frac": lane_ops / VALU_PEAK_LANE_OPS}
+    if tuned:
+        out["autotune"] = tuned
     out["setup_io"] = setup_and_io(M, sim, sc, dt, host_gen_ms, upload_ms, volumes_ms)
     if args.copy_probe:
         sim.copy_probe(1 << 30)
