@@ -224,21 +224,27 @@ def test_two_snowballs_one_and_ten_substeps():
 
 
 def test_resort_interval_does_not_change_results():
-    """Re-binning every substep vs every 16: same physics (float atomics
-    aside), including particles that drift across block boundaries."""
+    """Re-binning every substep vs every 16 or 32, with the block pass on its
+    default width, on one workgroup that claims every home block in turn, or
+    on more workgroups than there are home blocks: same physics (float
+    atomics aside), including particles that drift across block boundaries."""
     sc = scenes.two_snowballs(n_total=32768, grid=64, dt=2e-4)
     out = []
-    for every in (1, 16):
+    for every, width in ((1, None), (16, None), (32, 1), (32, 4096)):
         prm = sc["params"]
         p = M.default_params(sc["dims"], sc["h"], bc_mode=prm["bc_mode"], theta_s=prm["theta_s"], resort_every=every)
         g = M.Simulation(p, sc["x"].shape[0])
+        if width is not None:
+            g.set_tuning(every, width)
         g.set_particles(sc["x"], sc["v"], sc["m"], sc["E0"])
         g.init_volumes()
         g.substeps(sc["dt"], 40)
         g.sync()
         out.append(g.particles())
-    for k in ("x", "v", "FE", "FP"):
-        assert rel_err(out[0][k], out[1][k]) < 1e-5, k
+        g.close()
+    for o in out[1:]:
+        for k in ("x", "v", "FE", "FP"):
+            assert rel_err(out[0][k], o[k]) < 1e-5, k
 
 
 def test_graph_replay_matches_direct_launches():
