@@ -144,7 +144,7 @@ def main():
     if not args.no_autotune:
         # mpm_autotune times each candidate from the current state and
         # restores it; the timed substeps below start from the same state
-        tuned = dict(ms_per_substep=sim.autotune(dt, 64))
+        tuned = dict(ms_per_substep=sim.autotune(dt, 128))
         tuned["resort_every"], tuned["block_grid"] = sim.tuning()
 
     sim.substeps(dt, args.warmup)

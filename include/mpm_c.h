@@ -229,9 +229,9 @@ int mpm_create_from_snapshot(const char* path, mpm_ctx** out, int64_t* step);
 
 /* ---- launch tuning (no reference counterpart).  resort_every: substeps
  * between re-binnings; block_grid: workgroups of the persistent block pass.
- * mpm_autotune times each candidate (resort_every in {8, 16, 32} x
+ * mpm_autotune times each candidate (resort_every in {8, 16, 32, 64} x
  * block_grid in {1024, 2048, 4096}, capped at the block count) over at least
- * `steps` substeps, rounded up to whole re-bin cycles and at least 64, from
+ * `steps` substeps, rounded up to whole re-bin cycles and at least 128, from
  * the current state; it restores that state, keeps the fastest and returns
  * its time per substep. */
 int mpm_set_tuning(mpm_ctx* ctx, int32_t resort_every, int32_t block_grid);

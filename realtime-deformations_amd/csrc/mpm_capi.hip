@@ -817,7 +817,7 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms) {
   hipEvent_t e0, e1;
   MPM_HIP(hipEventCreate(&e0));
   MPM_HIP(hipEventCreate(&e1));
-  const int resorts[] = {8, 16, 32};
+  const int resorts[] = {8, 16, 32, 64};
   const int64_t wmax = std::max<int64_t>(1, ctx->nblocks);
   const int widths[] = {(int)std::min<int64_t>(wmax, 1024), (int)std::min<int64_t>(wmax, 2048),
                         (int)std::min<int64_t>(wmax, 4096)};
@@ -825,7 +825,7 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms) {
   // every candidate is timed over whole re-bin cycles of every interval, and
   // over two of the longest, so the re-bin cost and the drift out of the
   // halo margins between re-bins are both counted
-  const int cycle = 32;
+  const int cycle = 64;
   const int window = (std::max(steps, 2 * cycle) + cycle - 1) / cycle * cycle;
   double best = 1e300;
   int best_r = keep_r, best_w = keep_w, rc = MPM_OK;

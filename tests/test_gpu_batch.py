@@ -130,7 +130,7 @@ def test_autotune_keeps_state_and_picks_a_candidate():
     ms = b.autotune(1e-4, steps=8)
     assert ms > 0.0
     r, g = b.tuning()
-    assert r in (8, 16, 32) and g >= 1
+    assert r in (8, 16, 32, 64) and g >= 1
     after = b.particles()
     for k in before:
         np.testing.assert_array_equal(before[k], after[k])
