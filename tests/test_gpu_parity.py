@@ -224,13 +224,13 @@ def test_two_snowballs_one_and_ten_substeps():
 
 
 def test_resort_interval_does_not_change_results():
-    """Re-binning every substep vs every 16 or 32, with the block pass on its
+    """Re-binning every substep vs every 16, 32 or 64, with the block pass on its
     default width, on one workgroup that claims every home block in turn, or
     on more workgroups than there are home blocks: same physics (float
     atomics aside), including particles that drift across block boundaries."""
     sc = scenes.two_snowballs(n_total=32768, grid=64, dt=2e-4)
     out = []
-    for every, width in ((1, None), (16, None), (32, 1), (32, 4096)):
+    for every, width in ((1, None), (16, None), (32, 1), (32, 4096), (64, 1024)):
         prm = sc["params"]
         p = M.default_params(sc["dims"], sc["h"], bc_mode=prm["bc_mode"], theta_s=prm["theta_s"], resort_every=every)
         g = M.Simulation(p, sc["x"].shape[0])
