@@ -1,6 +1,6 @@
 """GPU parity: the HIP library (through its C ABI) against the oracle.
 
-Tolerances (north star): grid mass and momentum and particle x, v, C, F_E,
+Tolerances: grid mass and momentum and particle x, v, C, F_E,
 F_P and J_P within 1e-5 norm-relative after one substep, within 1e-3 after
 100 substeps.  The deformation-gradient update on identical inputs is checked
 bit-exactly (same FP32 operation order, correctly rounded division and sqrt).
