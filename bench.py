@@ -336,9 +336,11 @@ def bench_batch(args):
 
 def bench_adjoint(args):
     """16 snowballs of 32,768 particles on 64^3 (TEXTBOOK composition, domain
-    walls, dt 1e-4): `steps` substeps forward with checkpoints every 20, then
-    the reverse sweep for a synthetic loss on the final positions and
-    velocities.  A step is one forward plus one backward substep."""
+    walls, dt 1e-4): `steps` substeps forward, keeping checkpoints at the
+    interval the library chooses (every substep when the records fit in
+    device memory), then the reverse sweep for a synthetic loss on the final
+    positions and velocities; the same with checkpoints every 20 is reported
+    beside it.  A step is one forward plus one backward substep."""
     import numpy as np
     import mpm_amd as M
     from mpm_amd import scenes
@@ -364,7 +366,7 @@ def bench_adjoint(args):
     # warm-up: each timed configuration once (the library keeps its
     # checkpoint, record and workspace buffers between calls); the backward
     # leaves the context at the initial state
-    for every in (20, 1):
+    for every in (0, 20):
         b.adjoint_forward(dt, steps, every)
         b.adjoint_backward(gx, gv)
     b.sync()
@@ -376,31 +378,29 @@ def bench_adjoint(args):
         b.scene_set_particles(i, s["x"], s["v"], s["m"], s["E0"])
     b.init_volumes()
     b.sync()
-    t0 = time.perf_counter()
-    b.adjoint_forward(dt, steps, 20)
-    b.sync()
-    fwd_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    gx0, gv0, gE0 = b.adjoint_backward(gx, gv)
-    b.sync()
-    bwd_s = time.perf_counter() - t0
-    # the same run keeping every substep's record in HBM (checkpoint_every = 1:
-    # no segment re-run in the backward)
-    b.sync()
-    t0 = time.perf_counter()
-    b.adjoint_forward(dt, steps, 1)
-    b.sync()
-    fwd1_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    gx1, gv1, gE1 = b.adjoint_backward(gx, gv)
-    b.sync()
-    bwd1_s = time.perf_counter() - t0
+
+    def run(every):
+        t0 = time.perf_counter()
+        b.adjoint_forward(dt, steps, every)
+        b.sync()
+        f = time.perf_counter() - t0
+        chosen = b.adjoint_interval()
+        t0 = time.perf_counter()
+        g = b.adjoint_backward(gx, gv)
+        b.sync()
+        return f, time.perf_counter() - t0, chosen, g
+
+    # the library's choice (checkpoint_every = 0): every substep's record
+    # when the records fit in HBM, as they do here; then checkpoints every 20
+    fwd_s, bwd_s, chosen, (gx0, gv0, gE0) = run(0)
+    fwd20_s, bwd20_s, _, (gx1, gv1, gE1) = run(20)
     b.close()
 
     def rel(a, r):
         return float(np.linalg.norm(a.astype(np.float64) - r) / max(np.linalg.norm(r.astype(np.float64)), 1e-30))
+    cells = sum(int(np.prod(sc["dims"])) for sc in scs)
     out = {
-        "metric": "fwd+bwd substeps/s, 16 x 32,768 particles on 64^3 (checkpoints every 20)",
+        "metric": "fwd+bwd substeps/s, 16 x 32,768 particles on 64^3 (checkpoint interval chosen by the library)",
         "value": steps / (fwd_s + bwd_s),
         "unit": "substeps/s",
         "n_gpus": 1,
@@ -412,22 +412,22 @@ def bench_adjoint(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "16 snowballs x 32,768 particles, 64^3, dt 1e-4, TEXTBOOK, checkpoints every 20",
+        "config": {"workload": "16 snowballs x 32,768 particles, 64^3, dt 1e-4, TEXTBOOK",
                    "particles": n, "scenes": len(scs)},
+        "checkpoint_every": chosen,
         "forward_ms_per_substep": fwd_s / steps * 1e3,
         "backward_ms_per_substep": bwd_s / steps * 1e3,
         "plain_forward_ms_per_substep": plain_s / steps * 1e3,
-        "fwd_plus_bwd_over_checkpointed_fwd": (fwd_s + bwd_s) / fwd_s,
         "fwd_plus_bwd_over_plain_fwd": (fwd_s + bwd_s) / plain_s,
         "particle_substeps_per_s_fwd_bwd": n * steps / (fwd_s + bwd_s),
-        "keep_every_substep": {
-            "forward_ms_per_substep": fwd1_s / steps * 1e3,
-            "backward_ms_per_substep": bwd1_s / steps * 1e3,
-            "fwd_plus_bwd_over_plain_fwd": (fwd1_s + bwd1_s) / plain_s,
-            # state rows, room for every grid node's P2G sums, the block list
-            "record_bytes_per_substep": 132 * n + 16 * sum(int(np.prod(sc["dims"])) for sc in scs)
-            + 4 * (sum(int(np.prod(sc["dims"])) for sc in scs) // 64 + 1),
-            "grad_rel_diff_vs_checkpoints": {"x0": rel(gx1, gx0), "v0": rel(gv1, gv0), "E0": rel(gE1, gE0)},
+        # state rows, room for every grid node's P2G sums, the block list
+        "record_bytes_per_substep": 132 * n + 16 * cells + 4 * (cells // 64 + 1),
+        "checkpoints_every_20": {
+            "forward_ms_per_substep": fwd20_s / steps * 1e3,
+            "backward_ms_per_substep": bwd20_s / steps * 1e3,
+            "fwd_plus_bwd_over_plain_fwd": (fwd20_s + bwd20_s) / plain_s,
+            "fwd_plus_bwd_over_checkpointed_fwd": (fwd20_s + bwd20_s) / fwd20_s,
+            "grad_rel_diff": {"x0": rel(gx1, gx0), "v0": rel(gv1, gv0), "E0": rel(gE1, gE0)},
         },
         "grad_finite": bool(np.isfinite(gx0).all() and np.isfinite(gv0).all() and np.isfinite(gE0).all()),
     }

@@ -244,7 +244,9 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms_per_subs
  * memory: 132 bytes per particle per checkpoint; checkpoint_every = 1
  * instead keeps every substep's record, 132 bytes per particle plus room for
  * 16 bytes per grid node and 4 per grid block per substep, and the backward
- * re-runs nothing).  Checkpoints, records and the backward's workspaces stay
+ * re-runs nothing; checkpoint_every = 0 chooses: 1 when those records fit in
+ * half of the free device memory, else 20).  mpm_adjoint_interval reports
+ * the interval the last forward used (0 before any).  Checkpoints, records and the backward's workspaces stay
  * allocated for the next call of the same shape, which reuses them; records
  * are released by a checkpoint_every = 1 call of another length, everything
  * by mpm_destroy.
@@ -256,6 +258,7 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms_per_subs
  * walls only (MPM_ERR_STATE otherwise); V0 is held fixed, so dL/dx0 leaves
  * out V0's dependence on the initial positions. */
 int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t checkpoint_every);
+int mpm_adjoint_interval(mpm_ctx* ctx, int32_t* checkpoint_every);
 int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* gx0, float* gv0, float* gE0);
 
 /* ---- measurement (no reference counterpart: the reference times stages

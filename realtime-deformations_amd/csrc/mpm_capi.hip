@@ -1142,7 +1142,7 @@ void adjoint_segment(mpm_ctx* ctx, float dt, int ns, const float* seg, float4* g
 extern "C" {
 
 int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
-  if (check_ctx(ctx) || nsteps < 1 || every < 1) return MPM_ERR_ARG;
+  if (check_ctx(ctx) || nsteps < 1 || every < 0) return MPM_ERR_ARG;
   if (int rc = adjoint_supported(ctx)) return rc;
   const int64_t sd = ctx->stride;
   ctx->adj_steps = 0;
@@ -1150,10 +1150,24 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
   // call and reused when the shape matches
   const int L = std::max(1, ctx->resort_every);
   std::vector<int> seg_ns;
-  if (every == 1)
-    for (int done = 0; done < nsteps; done += L) seg_ns.push_back(std::min(L, nsteps - done));
+  for (int done = 0; done < nsteps; done += L) seg_ns.push_back(std::min(L, nsteps - done));
   bool same = seg_ns.size() == ctx->adj_segs.size();
   for (size_t i = 0; same && i < seg_ns.size(); ++i) same = ctx->adj_segs[i].ns == seg_ns[i];
+  if (every == 0) {
+    // automatic: keep every substep when its records fit in half of the free
+    // device memory (or are already held for this shape), else checkpoints
+    // every 20 substeps
+    size_t free_b = 0, total_b = 0;
+    const double per_step = (double)sizeof(float) * NA * sd + (double)sizeof(float4) * ctx->nnodes +
+                            (double)sizeof(int) * (ctx->nblocks + 1);
+    const double need = per_step * (nsteps + (double)seg_ns.size()) + (double)sizeof(float) * NA * sd;
+    every = (same && !ctx->adj_segs.empty()) ||
+                    (hipMemGetInfo(&free_b, &total_b) == hipSuccess && need < 0.5 * (double)free_b)
+                ? 1
+                : 20;
+  }
+  if (every != 1) seg_ns.clear();
+  same = same && every == 1;
   if (every == 1 && !same) free_segments(ctx);  // other calls leave them for the next keep-all call
   if (!ctx->adj_const) MPM_HIP(hipMalloc(&ctx->adj_const, sizeof(float) * 3 * sd));
   if (!ctx->adj_inv) MPM_HIP(hipMalloc(&ctx->adj_inv, sizeof(int) * (size_t)std::max(1, ctx->n)));
@@ -1206,6 +1220,12 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
   ctx->adj_steps = nsteps;
   ctx->adj_every = every;
   MPM_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
+int mpm_adjoint_interval(mpm_ctx* ctx, int32_t* every) {
+  if (check_ctx(ctx) || !every) return MPM_ERR_ARG;
+  *every = ctx->adj_steps > 0 ? ctx->adj_every : 0;
   return MPM_OK;
 }
 

@@ -89,7 +89,7 @@ EXPORTS = [
     "mpm_count_active", "mpm_debug_bin", "mpm_debug_blocks", "mpm_copy_probe", "mpm_use_graphs",
     "mpm_create_batch", "mpm_num_scenes", "mpm_scene_set_particles", "mpm_scene_set_state",
     "mpm_scene_set_colliders", "mpm_scene_get_particles", "mpm_scene_get_grid", "mpm_scene_status",
-    "mpm_set_tuning", "mpm_get_tuning", "mpm_autotune", "mpm_adjoint_forward", "mpm_adjoint_backward",
+    "mpm_set_tuning", "mpm_get_tuning", "mpm_autotune", "mpm_adjoint_forward", "mpm_adjoint_interval", "mpm_adjoint_backward",
     "mpm_scene_generate", "mpm_generate", "mpm_scene_get_material", "mpm_get_positions", "mpm_positions_begin",
     "mpm_positions_wait", "mpm_snapshot_save", "mpm_snapshot_load", "mpm_create_from_snapshot",
 ]
@@ -134,6 +134,7 @@ def lib():
         L.mpm_get_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.mpm_autotune.argtypes = [vp, C.c_float, C.c_int32, C.POINTER(C.c_double)]
         L.mpm_adjoint_forward.argtypes = [vp, C.c_float, C.c_int32, C.c_int32]
+        L.mpm_adjoint_interval.argtypes = [vp, C.POINTER(C.c_int32)]
         L.mpm_adjoint_backward.argtypes = [vp, _F, _F, _F, _F, _F]
         L.mpm_profile_end.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.mpm_count_active.argtypes = [vp, C.POINTER(C.c_int64)]
@@ -415,9 +416,17 @@ class Simulation:
         return ms.value
 
     # ---- reverse mode
-    def adjoint_forward(self, dt, nsteps, checkpoint_every=20):
-        """nsteps substeps from the current state, keeping checkpoints."""
+    def adjoint_forward(self, dt, nsteps, checkpoint_every=0):
+        """nsteps substeps from the current state, keeping checkpoints every
+        `checkpoint_every` substeps (1: every substep's record; 0: the library
+        chooses 1 when the records fit in device memory, else 20)."""
         self._check(self.L.mpm_adjoint_forward(self.ctx, float(dt), int(nsteps), int(checkpoint_every)))
+
+    def adjoint_interval(self):
+        """Checkpoint interval the last adjoint_forward used (0 before any)."""
+        e = C.c_int32()
+        self._check(self.L.mpm_adjoint_interval(self.ctx, C.byref(e)))
+        return e.value
 
     def adjoint_backward(self, gx=None, gv=None):
         """dL/dx, dL/dv of the final particles -> (dL/dx0, dL/dv0, dL/dE0);

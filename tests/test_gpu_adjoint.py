@@ -143,6 +143,29 @@ def test_keep_every_substep_matches_checkpoints(case):
     g.close()
 
 
+def test_automatic_interval_keeps_every_substep_when_it_fits(case):
+    """checkpoint_every = 0 lets the library choose: at this size every
+    substep's record fits in device memory, so it keeps them all
+    (mpm_adjoint_interval reports 1) and the gradients are those of the
+    checkpointed sweep, to the tolerances measured across intervals; an
+    explicit interval is reported as given."""
+    sc = case["sc"]
+    g = _gpu(sc)
+    assert g.adjoint_interval() == 0
+    g.set_particles(sc["x"], sc["v"], sc["m"], sc["E0"])
+    g.init_volumes()
+    g.adjoint_forward(DT, STEPS, 0)
+    assert g.adjoint_interval() == 1
+    gx0, gv0, gE0 = g.adjoint_backward(case["a"], case["b"])
+    for got, ref, tol in ((gx0, case["gx0"], 1e-5), (gv0, case["gv0"], 1e-5), (gE0, case["gE0"], 1e-3)):
+        err = np.linalg.norm(got.astype(np.float64) - ref) / np.linalg.norm(ref.astype(np.float64))
+        assert err < tol, (err, tol)
+    g.adjoint_forward(DT, STEPS, 5)
+    assert g.adjoint_interval() == 5
+    g.adjoint_backward(case["a"], case["b"])
+    g.close()
+
+
 def _scene_b():
     sc = scenes.snowball(seed=22, n=1500, grid=40, radius=0.1, centre=(0.5, 0.55, 0.5), v0=(0.5, -0.5, 0.0), E=1.5e5)
     c = np.float32([0.5, 0.55, 0.5])
