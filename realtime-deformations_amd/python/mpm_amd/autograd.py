@@ -3,7 +3,8 @@
     Rollout.apply(sim, dt, nsteps, every, x0, v0, E0) -> (x_T, v_T)
 
 runs `nsteps` substeps of the context from (x0, v0, E0) (float32 CPU
-tensors; x0, v0: n x 3, E0: n) with checkpoints every `every` substeps, and
+tensors; x0, v0: n x 3, E0: n) with checkpoints every `every` substeps (0:
+the library's choice, every substep when the records fit in device memory), and
 backpropagates dL/dx_T, dL/dv_T into dL/dx0, dL/dv0, dL/dE0 through
 mpm_adjoint_backward.  The context's initial volumes are computed from x0 on
 every forward and held fixed in the derivative.  Used by the gradient checks
