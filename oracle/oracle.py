@@ -71,6 +71,7 @@ def lib():
         L.orc_get_particles.argtypes = [C.c_void_p] + [_D] * 6
         L.orc_get_grid.argtypes = [C.c_void_p, _D, _D, _D, C.POINTER(C.c_uint8)]
         L.orc_num_active.argtypes = [C.c_void_p]
+        L.orc_set_grid.argtypes = [C.c_void_p, _D, _D]
         L.orc_weight_f.restype = C.c_float
         L.orc_weight_f.argtypes = [C.c_float]
         L.orc_weight_d.restype = C.c_double
@@ -174,6 +175,13 @@ class Sim:
                    FP=np.zeros((n, 9)), V0=np.zeros(n))
         self.L.orc_get_particles(self.h, *[dptr(out[k]) for k in ("x", "v", "B", "FE", "FP", "V0")])
         return out
+
+    def set_grid(self, m, vel):
+        """Dense node masses (cells) and velocities (cells x 3); the used set
+        becomes the nodes of non-zero mass."""
+        m = np.ascontiguousarray(m, np.float64)
+        vel = np.ascontiguousarray(vel, np.float64)
+        self.L.orc_set_grid(self.h, dptr(m), dptr(vel))
 
     def grid(self):
         cells = int(np.prod(self.dims))

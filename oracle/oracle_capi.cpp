@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only (see mpm_oracle.hpp).  A C interface over
 // the CPU restatement for the Python tests (ctypes) and for the CPU baseline
 // timed by bench.py.  Never linked into the product.
+#include <type_traits>
 #include <chrono>
 #include <cstdio>
 #include <memory>
@@ -176,6 +177,27 @@ int orc_get_grid(void* hp, double* m, double* vel, double* force, unsigned char*
     get_grid(h->d, m, vel, force, active);
   else
     get_grid(h->f, m, vel, force, active);
+  return 0;
+}
+// Dense grid masses and velocities from the caller (index i*J*K + j*K + k);
+// the used set becomes the nodes of non-zero mass, as after p2g.  Lets a
+// known field (e.g. an affine velocity field) drive g2p directly.
+int orc_set_grid(void* hp, const double* m, const double* vel) {
+  auto* h = static_cast<Handle*>(hp);
+  auto put = [&](auto& s) {
+    using T = std::decay_t<decltype(s.gm[0])>;
+    s.used.clear();
+    for (size_t c = 0; c < s.gm.size(); ++c) {
+      s.gm[c] = (T)m[c];
+      s.gvel[c] = {(T)vel[3 * c], (T)vel[3 * c + 1], (T)vel[3 * c + 2]};
+      s.gforce[c] = {T(0), T(0), T(0)};
+      if (s.gm[c] != T(0)) s.used.push_back((int64_t)c);
+    }
+  };
+  if (h->dbl)
+    put(h->d);
+  else
+    put(h->f);
   return 0;
 }
 int orc_num_active(void* hp) {

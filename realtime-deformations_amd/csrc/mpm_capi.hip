@@ -251,27 +251,25 @@ namespace mpm {
 int upload_field(mpm_ctx* ctx, int scene, int field, const float* host, int comps) {
   const Scene& S = ctx->scenes[scene];
   if (!host || S.np == 0) return MPM_OK;
-  float* tmp = nullptr;
+  DevBuf<float> tmp;
   const size_t bytes = sizeof(float) * (size_t)S.np * comps;
-  MPM_HIP(hipMalloc(&tmp, bytes));
-  MPM_HIP(hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, ctx->stream));
-  launch_scatter_field(ctx->P + (int64_t)field * ctx->stride, ctx->stride, tmp, ctx->pid, S.p0, S.np, comps,
+  MPM_HIP(tmp.alloc(bytes));
+  MPM_HIP(hipMemcpyAsync(tmp.p, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  launch_scatter_field(ctx->P + (int64_t)field * ctx->stride, ctx->stride, tmp.p, ctx->pid, S.p0, S.np, comps,
                        ctx->stream);
   MPM_HIP(hipStreamSynchronize(ctx->stream));
-  MPM_HIP(hipFree(tmp));
   return MPM_OK;
 }
 int download_field(mpm_ctx* ctx, int scene, int field, float* host, int comps) {
   const Scene& S = ctx->scenes[scene];
   if (!host || S.np == 0) return MPM_OK;
-  float* tmp = nullptr;
+  DevBuf<float> tmp;
   const size_t bytes = sizeof(float) * (size_t)S.np * comps;
-  MPM_HIP(hipMalloc(&tmp, bytes));
-  launch_gather_field(tmp, ctx->P + (int64_t)field * ctx->stride, ctx->stride, ctx->pid, S.p0, S.np, comps,
+  MPM_HIP(tmp.alloc(bytes));
+  launch_gather_field(tmp.p, ctx->P + (int64_t)field * ctx->stride, ctx->stride, ctx->pid, S.p0, S.np, comps,
                       ctx->stream);
-  MPM_HIP(hipMemcpyAsync(host, tmp, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  MPM_HIP(hipMemcpyAsync(host, tmp.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
   MPM_HIP(hipStreamSynchronize(ctx->stream));
-  MPM_HIP(hipFree(tmp));
   return MPM_OK;
 }
 int check_scene_positions(mpm_ctx* ctx, int scene, const float* x) { return check_positions(ctx, scene, x); }
@@ -799,24 +797,40 @@ int mpm_get_tuning(mpm_ctx* ctx, int32_t* resort_every, int32_t* block_grid) {
 
 // Times `steps` substeps for every candidate (re-bin interval x block-pass
 // width) from the same starting state, keeps the fastest, and puts the
-// particle state back as it was.
+// particle state back as it was.  Status flags raised by the candidate runs
+// (which run ahead of the caller's state) are read and cleared here, so they
+// never surface later as errors of the restored state; a flagged run makes
+// the tuning fail with that status and keeps the caller's settings.
 int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms) {
   if (check_ctx(ctx) || steps < 1) return MPM_ERR_ARG;
   if (!ctx->particles_set) {
     ctx->err = "no particles set";
     return MPM_ERR_STATE;
   }
+  if (int rc = read_status(ctx)) return rc;  // the caller's own pending flags first
   const size_t pbytes = sizeof(float) * (size_t)NFIELDS * (size_t)ctx->stride;
-  float* snapP = nullptr;
-  int* snapId = nullptr;
-  MPM_HIP(hipMalloc(&snapP, pbytes));
-  MPM_HIP(hipMalloc(&snapId, sizeof(int) * (size_t)std::max(1, ctx->n)));
-  MPM_HIP(hipMemcpyAsync(snapP, ctx->P, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
-  MPM_HIP(hipMemcpyAsync(snapId, ctx->pid, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
+  DevBuf<float> snapP;
+  DevBuf<int> snapId;
+  MPM_HIP(snapP.alloc(pbytes));
+  MPM_HIP(snapId.alloc(sizeof(int) * (size_t)std::max(1, ctx->n)));
+  MPM_HIP(hipMemcpyAsync(snapP.p, ctx->P, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
+  MPM_HIP(hipMemcpyAsync(snapId.p, ctx->pid, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
   float* const snapBuf = ctx->P;  // the buffer the snapshot belongs in
-  hipEvent_t e0, e1;
-  MPM_HIP(hipEventCreate(&e0));
-  MPM_HIP(hipEventCreate(&e1));
+  // puts the snapshot back into the buffer it came from
+  auto restore = [&]() -> int {
+    if (ctx->P != snapBuf) {
+      std::swap(ctx->P, ctx->P2);
+      std::swap(ctx->pid, ctx->pid2);
+    }
+    MPM_HIP(hipMemcpyAsync(ctx->P, snapP.p, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
+    MPM_HIP(hipMemcpyAsync(ctx->pid, snapId.p, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->since_sort = -1;
+    ctx->gacc_clean = false;
+    return MPM_OK;
+  };
+  EventGuard e0, e1;
+  MPM_HIP(hipEventCreate(&e0.e));
+  MPM_HIP(hipEventCreate(&e1.e));
   const int resorts[] = {8, 16, 32, 64};
   const int64_t wmax = std::max<int64_t>(1, ctx->nblocks);
   const int widths[] = {(int)std::min<int64_t>(wmax, 1024), (int)std::min<int64_t>(wmax, 2048),
@@ -828,46 +842,38 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms) {
   const int cycle = 64;
   const int window = (std::max(steps, 2 * cycle) + cycle - 1) / cycle * cycle;
   double best = 1e300;
-  int best_r = keep_r, best_w = keep_w, rc = MPM_OK;
-  for (int r : resorts) {
-    for (int w : widths) {
-      // restore the snapshot into the buffer it came from
-      if (ctx->P != snapBuf) {
-        std::swap(ctx->P, ctx->P2);
-        std::swap(ctx->pid, ctx->pid2);
+  int best_r = keep_r, best_w = keep_w;
+  auto candidates = [&]() -> int {
+    for (int r : resorts)
+      for (int w : widths) {
+        if (int rc = restore()) return rc;
+        ctx->resort_every = r;
+        ctx->block_grid = w;
+        MPM_HIP(hipEventRecord(e0.e, ctx->stream));
+        if (int rc = mpm_substeps(ctx, dt, window)) return rc;
+        MPM_HIP(hipEventRecord(e1.e, ctx->stream));
+        MPM_HIP(hipEventSynchronize(e1.e));
+        float ms = 0.0f;
+        MPM_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
+        if (ms < best) {
+          best = ms;
+          best_r = r;
+          best_w = w;
+        }
       }
-      MPM_HIP(hipMemcpyAsync(ctx->P, snapP, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
-      MPM_HIP(hipMemcpyAsync(ctx->pid, snapId, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
-      ctx->since_sort = -1;
-      ctx->resort_every = r;
-      ctx->block_grid = w;
-      MPM_HIP(hipEventRecord(e0, ctx->stream));
-      if ((rc = mpm_substeps(ctx, dt, window))) break;
-      MPM_HIP(hipEventRecord(e1, ctx->stream));
-      MPM_HIP(hipEventSynchronize(e1));
-      float ms = 0.0f;
-      MPM_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) {
-        best = ms;
-        best_r = r;
-        best_w = w;
-      }
-    }
-    if (rc) break;
+    return MPM_OK;
+  };
+  int rc = candidates();
+  // the caller's state back, whatever happened above
+  const int rrc = restore();
+  if (!rc) rc = rrc;
+  if (!rc) MPM_HIP(hipStreamSynchronize(ctx->stream));
+  const int src = read_status(ctx);  // flags of the candidate runs, cleared
+  if (!rc && src) {
+    rc = src;
+    ctx->err = "mpm_autotune: a candidate run raised a status flag (" + ctx->err +
+               "); tuning left unchanged, particle state restored";
   }
-  if (ctx->P != snapBuf) {
-    std::swap(ctx->P, ctx->P2);
-    std::swap(ctx->pid, ctx->pid2);
-  }
-  MPM_HIP(hipMemcpyAsync(ctx->P, snapP, pbytes, hipMemcpyDeviceToDevice, ctx->stream));
-  MPM_HIP(hipMemcpyAsync(ctx->pid, snapId, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
-  MPM_HIP(hipStreamSynchronize(ctx->stream));
-  ctx->since_sort = -1;
-  ctx->gacc_clean = false;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(snapP);
-  (void)hipFree(snapId);
   if (rc) {
     ctx->resort_every = keep_r;
     ctx->block_grid = keep_w;
@@ -974,22 +980,18 @@ int mpm_scene_get_grid(mpm_ctx* ctx, int32_t scene, float* m, float* vel, float*
   if (int rc = valid_scene(ctx, scene)) return rc;
   const Scene& S = ctx->scenes[scene];
   const int64_t cells = (int64_t)S.dims[0] * S.dims[1] * S.dims[2];
-  float *dm = nullptr, *dv = nullptr, *df = nullptr;
-  uint8_t* da = nullptr;
-  if (m) MPM_HIP(hipMalloc(&dm, sizeof(float) * cells));
-  if (vel) MPM_HIP(hipMalloc(&dv, sizeof(float) * 3 * cells));
-  if (force) MPM_HIP(hipMalloc(&df, sizeof(float) * 3 * cells));
-  if (active) MPM_HIP(hipMalloc(&da, cells));
-  launch_grid_to_dense(make_args(ctx, 0.0f), S, dm, dv, df, da, ctx->stream);
-  if (m) MPM_HIP(hipMemcpyAsync(m, dm, sizeof(float) * cells, hipMemcpyDeviceToHost, ctx->stream));
-  if (vel) MPM_HIP(hipMemcpyAsync(vel, dv, sizeof(float) * 3 * cells, hipMemcpyDeviceToHost, ctx->stream));
-  if (force) MPM_HIP(hipMemcpyAsync(force, df, sizeof(float) * 3 * cells, hipMemcpyDeviceToHost, ctx->stream));
-  if (active) MPM_HIP(hipMemcpyAsync(active, da, cells, hipMemcpyDeviceToHost, ctx->stream));
+  DevBuf<float> dm, dv, df;
+  DevBuf<uint8_t> da;
+  if (m) MPM_HIP(dm.alloc(sizeof(float) * cells));
+  if (vel) MPM_HIP(dv.alloc(sizeof(float) * 3 * cells));
+  if (force) MPM_HIP(df.alloc(sizeof(float) * 3 * cells));
+  if (active) MPM_HIP(da.alloc(cells));
+  launch_grid_to_dense(make_args(ctx, 0.0f), S, dm.p, dv.p, df.p, da.p, ctx->stream);
+  if (m) MPM_HIP(hipMemcpyAsync(m, dm.p, sizeof(float) * cells, hipMemcpyDeviceToHost, ctx->stream));
+  if (vel) MPM_HIP(hipMemcpyAsync(vel, dv.p, sizeof(float) * 3 * cells, hipMemcpyDeviceToHost, ctx->stream));
+  if (force) MPM_HIP(hipMemcpyAsync(force, df.p, sizeof(float) * 3 * cells, hipMemcpyDeviceToHost, ctx->stream));
+  if (active) MPM_HIP(hipMemcpyAsync(active, da.p, cells, hipMemcpyDeviceToHost, ctx->stream));
   MPM_HIP(hipStreamSynchronize(ctx->stream));
-  (void)hipFree(dm);
-  (void)hipFree(dv);
-  (void)hipFree(df);
-  (void)hipFree(da);
   return MPM_OK;
 }
 
@@ -1262,8 +1264,11 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
     MPM_HIP(hipMemcpyAsync(tmp, gv, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
     launch_aos_to_rows(Gc + (int64_t)AV * sd, tmp, n, 3, sd, s);
   }
+  // one binning per segment: slots stay fixed inside it; the caller's
+  // interval comes back on every exit, early error returns included
   const int keep_resort = ctx->resort_every;
-  ctx->resort_every = 1 << 30;  // one binning per segment: slots stay fixed inside it
+  ctx->resort_every = 1 << 30;
+  auto restore_resort = on_scope_exit([ctx, keep_resort] { ctx->resort_every = keep_resort; });
   if (every == 1) {
     // recorded segments: restore each one's slot order and block list
     for (int c = (int)ctx->adj_segs.size() - 1; c >= 0 && !rc; --c) {

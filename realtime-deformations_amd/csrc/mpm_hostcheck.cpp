@@ -2,8 +2,10 @@
 // that CPU tests can hold the weights, the constant divisions, the Jacobi SVD
 // and the deformation-gradient update to the oracle bit for bit.  Built as
 // lib/libmpm_hostcheck.so; not part of libmpm_hip.
+#include <algorithm>
 #include <cstring>
 
+#include "mpm_hostutil.hpp"
 #include "mpm_math.hpp"
 
 using namespace mpm;
@@ -75,6 +77,16 @@ int hc_fupdate(int compose, float dt, float h, double theta_c, double theta_s, c
   store9(fe, FEo);
   store9(fp, FPo);
   return ok ? 0 : 1;
+}
+
+// The byte ranges mpm_get_positions hands to nt copy threads for a read of
+// `bytes` bytes: begin[t], end[t] for t < nt.
+void hc_readback_parts(size_t bytes, unsigned nt, size_t* begin, size_t* end) {
+  const size_t part = readback_part_bytes(bytes, nt);
+  for (unsigned t = 0; t < nt; ++t) {
+    begin[t] = std::min(bytes, part * t);
+    end[t] = t + 1 == nt ? bytes : std::min(bytes, part * (t + 1));
+  }
 }
 
 }  // extern "C"

@@ -31,6 +31,34 @@ enum Field : int {
 
 enum StatusBits : int { ST_NONFINITE = 1, ST_BOUNDS = 2 };
 
+// A device allocation freed when the scope ends, error returns included.
+template <typename T> struct DevBuf {
+  T* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
+};
+// A HIP event destroyed when the scope ends.
+struct EventGuard {
+  hipEvent_t e = nullptr;
+  EventGuard() = default;
+  EventGuard(const EventGuard&) = delete;
+  EventGuard& operator=(const EventGuard&) = delete;
+  ~EventGuard() {
+    if (e) (void)hipEventDestroy(e);
+  }
+};
+// Runs f when the scope ends (restores context settings on early returns).
+template <typename F> struct ScopeExit {
+  F f;
+  ~ScopeExit() { f(); }
+};
+template <typename F> ScopeExit<F> on_scope_exit(F f) { return ScopeExit<F>{f}; }
+
 // One scene: its grid (dims, spacing, place in the shared grid arrays), its
 // particle slots, material and boundary conditions.  A context steps any
 // number of scenes together; kernels read a scene's entry per block or per

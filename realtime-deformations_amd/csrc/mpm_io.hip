@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "mpm_hostutil.hpp"
 #include "mpm_internal.hpp"
 
 using namespace mpm;
@@ -251,9 +252,12 @@ int mpm_get_positions(mpm_ctx* ctx, float* x) {
   if (nt == 1) {
     std::memcpy(x, view, bytes);
   } else {
-    const size_t part = (bytes / nt + 63) / 64 * 64;
+    // parts of ceil(bytes / nt) rounded up to 64 bytes: nt of them cover
+    // every byte, and the last one ends at `bytes` whatever the rounding
+    const size_t part = readback_part_bytes(bytes, nt);
     auto copy_part = [=](unsigned t) {
-      const size_t o = std::min(bytes, part * t), e = std::min(bytes, part * (t + 1));
+      const size_t o = std::min(bytes, part * t);
+      const size_t e = t + 1 == nt ? bytes : std::min(bytes, part * (t + 1));
       std::memcpy(reinterpret_cast<char*>(x) + o, reinterpret_cast<const char*>(view) + o, e - o);
     };
     std::vector<std::thread> pool;

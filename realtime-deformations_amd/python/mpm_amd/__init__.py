@@ -421,6 +421,10 @@ class Simulation:
         `checkpoint_every` substeps (1: every substep's record; 0: the library
         chooses 1 when the records fit in device memory, else 20)."""
         self._check(self.L.mpm_adjoint_forward(self.ctx, float(dt), int(nsteps), int(checkpoint_every)))
+        # a context holds one reverse-mode record at a time: each forward
+        # replaces it, and callers that keep a record for a later backward
+        # (autograd.Rollout) compare this counter to detect that
+        self.adjoint_generation = getattr(self, "adjoint_generation", 0) + 1
 
     def adjoint_interval(self):
         """Checkpoint interval the last adjoint_forward used (0 before any)."""

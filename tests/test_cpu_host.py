@@ -138,3 +138,50 @@ def test_binning_reference_is_a_stable_sort_with_column_groups():
         cols = np.array(order) & 15
         rr += [np.bincount(cols[g0:g0 + 16], minlength=16).max() - 1 for g0 in range(0, len(order), 16)]
     assert extra < 0.75 * float(np.mean(rr)), (extra, np.mean(rr))
+
+
+class _StubSim:
+    """Stands in for a device context: the reverse-mode record is the last
+    forward's tag; backward returns gradients tagged with it."""
+
+    def __init__(self, n):
+        self.n, self.rollout_mass, self.adjoint_generation = n, np.ones(n, np.float32), 0
+        self.record = None
+
+    def set_particles(self, x, v, m, E0=None):
+        self.x = np.asarray(x)
+
+    def init_volumes(self):
+        pass
+
+    def adjoint_forward(self, dt, nsteps, every=0):
+        self.record = float(self.x[0, 0])
+        self.adjoint_generation += 1
+
+    def particles(self):
+        return dict(x=self.x.astype(np.float32), v=np.zeros_like(self.x, np.float32))
+
+    def adjoint_backward(self, gx=None, gv=None):
+        g = np.full((self.n, 3), self.record, np.float32)
+        return g, g, np.full(self.n, self.record, np.float32)
+
+
+def test_rollout_backward_refuses_a_replaced_record():
+    """One reverse-mode record per context: a second Rollout on the same
+    context replaces the first one's record, so the first one's backward
+    raises instead of returning the second trajectory's gradients; a rollout
+    whose record is current differentiates normally."""
+    torch = pytest.importorskip("torch")
+    from mpm_amd.autograd import Rollout
+
+    sim = _StubSim(4)
+    E0 = torch.ones(4, requires_grad=True)
+    v0 = torch.zeros(4, 3, requires_grad=True)
+    xa = torch.full((4, 3), 1.0, requires_grad=True)
+    xb = torch.full((4, 3), 2.0, requires_grad=True)
+    outa = Rollout.apply(sim, 1e-4, 3, 0, xa, v0, E0)
+    outb = Rollout.apply(sim, 1e-4, 3, 0, xb, v0, E0)
+    with pytest.raises(RuntimeError, match="another adjoint forward"):
+        outa[0].sum().backward()
+    outb[0].sum().backward()
+    assert torch.all(xb.grad == 2.0)

@@ -133,3 +133,28 @@ def test_dp_inverse_pattern_all_spacings(hc):
     hc.hc_dinv_pattern.restype = C.c_int
     for h in (1 / 20, 0.05, 1 / 64, 1 / 96, 1 / 128, 1 / 256, 0.0123, 0.3, 2.0):
         assert hc.hc_dinv_pattern(h) == 1, h
+
+
+def test_positions_readback_parts_cover_every_byte(hc):
+    """mpm_get_positions splits the copy out of its pinned mirror over up to
+    8 threads: the parts must tile [0, bytes) exactly, start on 64-byte
+    boundaries and never run past the end, for every particle count, also
+    where bytes / nt is a multiple of 64 but bytes is not a multiple of nt
+    (n = 1,747,627 at 5 threads) and for n = 43 (mod 128) at 8 threads."""
+    hc.hc_readback_parts.argtypes = [C.c_size_t, C.c_uint, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    chunk = 4 << 20
+    ns = [1_747_627, 1_747_627 + 128, 4_194_304, 6_000_001, 128 * 20_000 + 43, 128 * 50_001 + 43, 349_526, 3]
+    ns += list(range(2_796_000, 2_796_400, 7))
+    for n in ns:
+        nbytes = 12 * n
+        nt = min(8, max(1, nbytes // chunk))
+        b = (C.c_size_t * nt)()
+        e = (C.c_size_t * nt)()
+        hc.hc_readback_parts(nbytes, nt, b, e)
+        b, e = list(b), list(e)
+        assert b[0] == 0 and e[-1] == nbytes, (n, nt, b, e)
+        for t in range(nt):
+            assert b[t] <= e[t] <= nbytes
+            assert b[t] % 64 == 0
+            if t:
+                assert b[t] == e[t - 1], (n, t)

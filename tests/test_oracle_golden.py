@@ -63,11 +63,14 @@ def test_collider_pose_inverse_and_sdf_bit_exact():
             assert O.sdf(c13[c], pts[c, j]) == GOLD["col_sdf"][c, j], (c, j)
 
 
-@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref needs the reference tree (this container only)")
 def test_fresh_reference_vectors_agree():
     """Beyond the committed fixtures: fresh random inputs through the
-    compiled reference math, where it is built."""
+    compiled reference math, where it is built.  The library is looked up
+    here, not at collection, so a run that deselects this test never loads
+    it."""
     R = O.ref()
+    if R is None:
+        pytest.skip("oracle/_ref needs the reference tree (this container only)")
     rng = np.random.default_rng(99)
     for _ in range(2000):
         M = (np.eye(3) + rng.normal(0, rng.choice([1e-6, 1e-3, 1e-1]), (3, 3))).astype(np.float32).ravel()

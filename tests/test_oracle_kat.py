@@ -299,3 +299,192 @@ def test_fp32_against_fp64_build(compose):
             else:
                 assert err < tol, (steps, k, err)
     print("FP32 vs FP64", "textbook" if compose else "literal", {f"{k}@{s}": f"{e:.2e}" for (s, k), e in errs.items()})
+
+
+# ---------------------------------------------------------------------------
+# Known answers for init_volumes (a3), the grid update's force term (a5) and
+# the orientation of the APIC affine term in P2G and G2P (a2, a8).
+
+def _grid_params(double_dims=(64, 64, 64), h=1 / 64, **kw):
+    return O.make_params(double_dims, h, **kw)
+
+
+def _exact_stencil(q: Fraction):
+    """Nodes base-1 .. base+2 of a coordinate q = x / h (grid units) and their
+    exact cubic B-spline weights."""
+    base = int(q)
+    return [(base - 1 + d, _exact_N(q - (base - 1 + d))) for d in range(4)]
+
+
+def test_initial_volumes_exact_rationals():
+    """V0 = m_p / rho_p with rho_p = sum_i m_i w_ip / h^3 and m_i = sum_q m_q w_iq
+    (material_point_method.cpp:131-142).  For positions whose x / h are
+    dyadic rationals every weight is an exact rational, so V0 is known
+    exactly: for one isolated particle V0 = h^3 / prod_axes sum_d N(u_d)^2,
+    independent of its mass; for particles sharing nodes the cross terms
+    enter through m_i.  A density built from w instead of w^2, a missing
+    h^3, or another particle's mass left out of m_i all fail."""
+    h = Fraction(1, 64)
+    rng = np.random.default_rng(11)
+    # one isolated particle, several sub-cell positions
+    for _ in range(6):
+        q = [Fraction(int(rng.integers(20, 40)) * 256 + int(rng.integers(0, 256)), 256) for _ in range(3)]
+        x = np.array([[float(c * h) for c in q]], np.float32)
+        assert all(Fraction(float(xi)) / h == qi for xi, qi in zip(x[0], q))  # positions exact in FP32
+        s_axes = [sum(w * w for _, w in _exact_stencil(c)) for c in q]
+        V0_exact = h ** 3 / (s_axes[0] * s_axes[1] * s_axes[2])
+        for dbl, tol in ((True, 1e-14), (False, 2e-7)):
+            s = O.Sim(_grid_params(), 1, double=dbl)
+            s.set_particles(x, np.zeros((1, 3), np.float32), np.float32([3.7e-4]), np.float32([1.4e5]))
+            s.stage(O.STAGE_P2G)
+            s.stage(O.STAGE_VOLUMES)
+            got = s.particles()["V0"][0]
+            assert got == pytest.approx(float(V0_exact), rel=tol), (dbl, q)
+    # five particles with overlapping stencils and different masses
+    qs = [[Fraction(30 * 64 + int(rng.integers(0, 192)), 64) for _ in range(3)] for _ in range(5)]
+    ms = [Fraction(int(rng.integers(1, 9)), 1000) for _ in range(5)]
+    gm = {}
+    for q, m in zip(qs, ms):
+        for i, wi in _exact_stencil(q[0]):
+            for j, wj in _exact_stencil(q[1]):
+                for k, wk in _exact_stencil(q[2]):
+                    gm[(i, j, k)] = gm.get((i, j, k), Fraction(0)) + m * wi * wj * wk
+    want = []
+    for q, m in zip(qs, ms):
+        rho = Fraction(0)
+        for i, wi in _exact_stencil(q[0]):
+            for j, wj in _exact_stencil(q[1]):
+                for k, wk in _exact_stencil(q[2]):
+                    rho += gm[(i, j, k)] * wi * wj * wk
+        want.append(m / (rho / h ** 3))
+    x = np.array([[float(c * h) for c in q] for q in qs], np.float32)
+    s = O.Sim(_grid_params(), 5, double=True)
+    s.set_particles(x, np.zeros((5, 3), np.float32), np.array([float(m) for m in ms], np.float32),
+                    np.full(5, 1.4e5, np.float32))
+    s.stage(O.STAGE_P2G)
+    s.stage(O.STAGE_VOLUMES)
+    # masses are FP32 values of the rationals: 1e-7 relative
+    np.testing.assert_allclose(s.particles()["V0"], [float(v) for v in want], rtol=2e-7)
+
+
+def _fixed_corotated_PFt(F, Fp, E, nu=0.2, xi=10.0):
+    """P(F) F^T of the hardened fixed-corotated snow model in FP64 (numpy
+    SVD polar factor), material_point_method.cpp:236-248."""
+    U, _, Vt = np.linalg.svd(F)
+    R = U @ Vt
+    if np.linalg.det(R) < 0:  # the reference's polar keeps det R = +1 only for det F > 0
+        R = -R
+    mu0, lam0 = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    hard = np.exp(xi * (1 - np.linalg.det(Fp)))
+    J = np.linalg.det(F)
+    P = 2 * mu0 * hard * (F - R) + lam0 * hard * (J - 1) * J * np.linalg.inv(F).T
+    return P @ F.T
+
+
+def test_grid_update_momentum_and_first_moment():
+    """Grid velocity update v_i += dt (f_i / m_i + g) (material_point_method.cpp:
+    256-262) after a P2G of particles at rest and the explicit forces:
+      sum_i m_i v_i         = dt M g                 (internal forces sum to 0)
+      sum_i m_i v_i x_i^T   = dt (-sum_p V0_p P_p F_p^T + g sum_p m_p x_p^T)
+    because sum_i w_ip (x_i - x_p)(x_i - x_p)^T = h^2/3 I for the cubic
+    B-spline and the force is f_i = -V0 D^-1 P F^T w_ip (x_i - x_p).  The
+    second identity holds only with the f / m term as written: a dropped
+    force term, f m instead of f / m, or a sign slip fail it."""
+    rng = np.random.default_rng(12)
+    n, h, dt = 40, 1 / 64, 1e-4
+    x = (0.5 + rng.uniform(-6, 6, (n, 3)) * h).astype(np.float32)
+    m = rng.uniform(1e-4, 3e-4, n).astype(np.float32)
+    E0 = rng.uniform(1e5, 2e5, n).astype(np.float32)
+    V0 = rng.uniform(1e-7, 3e-7, n).astype(np.float32)
+    _, FE, FP = warm_state(rng, n, scale=2e-2)
+    FE = from_math(to_math(FE) + rng.normal(0, 0.03, (n, 3, 3))).astype(np.float32)  # non-symmetric strain
+    p = O.make_params((64, 64, 64), h, E=1.4e5, nu=0.2, xi=10.0)
+    s = O.Sim(p, n, double=True)
+    s.set_particles(x, np.zeros((n, 3), np.float32), m, E0, FE=FE, FP=FP, V0=V0)
+    s.stage(O.STAGE_P2G)
+    s.stage(O.STAGE_FORCES)
+    s.stage(O.STAGE_GRID, dt)
+    G = s.grid()
+    idx = np.flatnonzero(G["m"] != 0)
+    ii, jj, kk = np.unravel_index(idx, (64, 64, 64))
+    xi = np.stack([ii, jj, kk], 1) * np.float64(np.float32(h))
+    mv = G["m"][idx, None] * G["vel"][idx]
+    # gravity and dt enter as FP32 inputs, exactly as the FP32 path sees them
+    g = np.array([0.0, np.float32(-9.8), 0.0], np.float64)
+    dt = np.float64(np.float32(dt))
+    xs, ms = x.astype(np.float64), m.astype(np.float64)
+    np.testing.assert_allclose(mv.sum(0), dt * ms.sum() * g, rtol=0, atol=1e-12 * np.abs(dt * ms.sum() * g).max())
+    S = sum(float(V0[p_]) * _fixed_corotated_PFt(to_math(FE[p_])[0].astype(np.float64),
+                                                 to_math(FP[p_])[0].astype(np.float64), float(E0[p_]))
+            for p_ in range(n))
+    want = dt * (-S + np.outer(g, (ms[:, None] * xs).sum(0)))
+    got = mv.T @ xi
+    # D^-1 is glm's inverse of h^2 * float(1/3): 3e-8 relative to 3 / h^2
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-7 * np.abs(want).max())
+    # the stress part alone (gravity's moment removed) is not small
+    assert np.abs(dt * S).max() > 0.05 * np.abs(want).max()
+
+
+def _nonsym(rng, scale):
+    A = rng.normal(0, scale, (3, 3))
+    assert np.abs(A - A.T).max() > 0.3 * scale
+    return A
+
+
+def test_p2g_reproduces_the_particle_affine_velocity():
+    """P2G of one particle with C = B D^-1 = A (material_point_method.cpp:115):
+    every node's velocity p_i / m_i equals v_p + A (x_i - x_p), for a
+    non-symmetric A, so a transposed B D^-1 (A^T) or a transposed product
+    fails."""
+    rng = np.random.default_rng(13)
+    h = np.float32(1 / 64)
+    Dm = np.float64(np.float32(1.0 / 3.0)) * np.float64(h) * np.float64(h)  # glm: mat3(1) * (1/3) * h * h
+    for _ in range(5):
+        A = _nonsym(rng, 20.0)
+        x = (0.5 + rng.uniform(-0.5, 0.5, (1, 3)) * h).astype(np.float32)
+        v = rng.normal(0, 1, (1, 3)).astype(np.float32)
+        B = from_math((A * Dm)[None]).astype(np.float64)
+        s = O.Sim(_grid_params(), 1, double=True)
+        # B as FP32 input: C reproduces float32(A Dm) / Dm
+        Bf = B.astype(np.float32)
+        s.set_particles(x, v, np.float32([2e-4]), np.float32([1.4e5]), B=Bf)
+        s.stage(O.STAGE_P2G)
+        G = s.grid()
+        idx = np.flatnonzero(G["m"] != 0)
+        assert idx.size == 64
+        ii, jj, kk = np.unravel_index(idx, (64, 64, 64))
+        d = np.stack([ii, jj, kk], 1) * np.float64(h) - x.astype(np.float64)
+        Aeff = to_math(Bf.astype(np.float64))[0] / Dm
+        want = v.astype(np.float64) + d @ Aeff.T
+        np.testing.assert_allclose(G["vel"][idx], want, rtol=0, atol=1e-12 * np.abs(want).max())
+        wrong = v.astype(np.float64) + d @ Aeff
+        assert np.abs(G["vel"][idx] - wrong).max() > 1e3 * np.abs(G["vel"][idx] - want).max()
+
+
+def test_g2p_recovers_an_affine_grid_field():
+    """G2P (material_point_method.cpp:332-342) of a grid velocity field
+    v_i = b + A x_i: v_p = b + A x_p (linear reproduction) and, with
+    B_p = sum_i w_ip v_i (x_i - x_p)^T, B_p D^-1 = A since
+    sum_i w_ip (x_i - x_p)(x_i - x_p)^T = D = h^2/3 I.  A non-symmetric A
+    makes a transposed outer product (x_i - x_p) v_i^T give A^T instead."""
+    rng = np.random.default_rng(14)
+    dims = (64, 64, 64)
+    h = 1 / 64
+    ii, jj, kk = np.meshgrid(*[np.arange(d) for d in dims], indexing="ij")
+    xi = np.stack([ii, jj, kk], -1).reshape(-1, 3) * np.float64(np.float32(h))
+    for dbl, tol in ((True, 1e-10), (False, 2e-5)):
+        for _ in range(3):
+            A = _nonsym(rng, 5.0)
+            b = rng.normal(0, 1, 3)
+            vel = b + xi @ A.T
+            x = (0.5 + rng.uniform(-8, 8, (4, 3)) * h).astype(np.float32)
+            s = O.Sim(_grid_params(), 4, double=dbl)
+            s.set_particles(x, np.zeros((4, 3), np.float32), np.full(4, 1e-4, np.float32), np.full(4, 1.4e5, np.float32))
+            s.set_grid(np.ones(xi.shape[0]), vel)
+            s.stage(O.STAGE_G2P)
+            P = s.particles()
+            np.testing.assert_allclose(P["v"], b + x.astype(np.float64) @ A.T, rtol=0, atol=tol * 10)
+            C = to_math(P["B"]) * 3.0 / h ** 2
+            for p_ in range(4):
+                np.testing.assert_allclose(C[p_], A, rtol=0, atol=tol * np.abs(A).max() * 50)
+                assert np.abs(C[p_] - A.T).max() > 0.1 * np.abs(A).max()
