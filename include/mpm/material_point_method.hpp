@@ -280,6 +280,12 @@ struct LagrangeEulerView : Loggable {
   // Positions only, x y z per particle (what a viewer reads), from the
   // context's pinned host mirror: valid until the next getPositions().
   const float* getPositions();
+  // Device reads of the whole particle state made by getParticles() so far.
+  // The host mirror is re-read only after something changed the device state
+  // (a stage, a substep, the initial volumes), so a viewer that calls
+  // getParticles()[i] for every i each frame, as the reference's does
+  // (realtime-deformations/main.cpp:257-271), costs one read per frame.
+  int64_t particleReads() const { return reads; }
   mpm_ctx* context() { return ctx; }
   // Seed of the placement generator used by initializeParticles.
   uint64_t seed = 0x5eedULL;
@@ -296,6 +302,8 @@ struct LagrangeEulerView : Loggable {
   std::vector<float> positions;
   std::vector<Call> queue;
   std::vector<mpm_collider> colliders;
+  bool mirrorStale = true;  // device state changed since the mirror was read
+  int64_t reads = 0;
   void check(int rc, const char* what);
   void enqueue(int stage, ftype dt);
   void flush();

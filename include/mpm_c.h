@@ -34,9 +34,16 @@ enum {
   MPM_ERR_ARG = 1,       /* invalid argument                                   */
   MPM_ERR_HIP = 2,       /* HIP runtime failure (message has the HIP error)    */
   MPM_ERR_BOUNDS = 3,    /* a particle's 5^3 stencil leaves the grid           */
-  MPM_ERR_NONFINITE = 4, /* the SVD met a non-finite F (reference: SVD Error)  */
+  MPM_ERR_NONFINITE = 4, /* the SVD met a non-finite F (reference: SVD Error),
+                            or a particle's velocity or stress input is not
+                            finite on the fused path                           */
   MPM_ERR_STATE = 5,     /* call out of order (e.g. no particles set)          */
-  MPM_ERR_NOMEM = 6      /* device allocation failed                           */
+  MPM_ERR_NOMEM = 6,     /* device allocation failed                           */
+  MPM_ERR_CFL = 7        /* warning, reported by mpm_scene_status only: on the
+                            fused path a particle moved more than one cell in a
+                            substep (|v| dt > h), or the elastic wave of its
+                            hardened moduli crosses more than one cell
+                            (dt sqrt((lambda + 2 mu) V0 / m) > h)               */
 };
 
 /* F_E recomposition after the singular-value clamp
@@ -162,7 +169,12 @@ int mpm_scene_get_particles(mpm_ctx* ctx, int32_t scene, float* x, float* v, flo
 int mpm_scene_get_grid(mpm_ctx* ctx, int32_t scene, float* m, float* vel, float* force, uint8_t* active);
 /* Synchronises and returns, per scene, MPM_OK or the MPM_ERR_* its particles
  * raised since the last status read (a non-finite scene does not stop the
- * others); clears the flags. */
+ * others); clears the flags.  Precedence: MPM_ERR_NONFINITE (a non-finite
+ * velocity, stress input or F on the fused path, or the SVD's rejection),
+ * then MPM_ERR_BOUNDS, then the MPM_ERR_CFL warning.  Other calls report the
+ * first two as errors of the context; the CFL warning stays pending until
+ * this call reads it.  The flags are raised on the device each substep
+ * (an atomic OR only by a failing particle) and cost no synchronisation. */
 int mpm_scene_status(mpm_ctx* ctx, int32_t* status);
 
 /* ---- device scene generators.  Seeded particle placement written on the

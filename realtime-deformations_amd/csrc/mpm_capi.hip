@@ -115,26 +115,44 @@ int check_positions(mpm_ctx* ctx, int scene, const float* x) {
   return MPM_OK;
 }
 
-int status_code(int st) { return st & ST_NONFINITE ? MPM_ERR_NONFINITE : (st ? MPM_ERR_BOUNDS : MPM_OK); }
+int status_code(int st) {
+  if (st & ST_NONFINITE) return MPM_ERR_NONFINITE;
+  if (st & ST_BOUNDS) return MPM_ERR_BOUNDS;
+  return (st & ST_CFL) ? MPM_ERR_CFL : MPM_OK;
+}
 
-// Per-scene device status words: returns (and clears) them; the first failing
-// scene sets the context error.
-int read_status(mpm_ctx* ctx, std::vector<int>* per_scene = nullptr) {
+// Per-scene device status words.  The error bits (non-finite, bounds) of
+// every scene are returned as the first failing scene's code, with the
+// context error set, and cleared; the CFL warning bit is left pending for
+// mpm_scene_status, unless `all` (that call) reads and clears every bit.
+int read_status(mpm_ctx* ctx, std::vector<int>* per_scene = nullptr, bool all = false) {
   std::vector<int> st(ctx->nscenes, 0);
   MPM_HIP(hipMemcpyAsync(st.data(), ctx->status, sizeof(int) * ctx->nscenes, hipMemcpyDeviceToHost, ctx->stream));
   MPM_HIP(hipStreamSynchronize(ctx->stream));
   if (per_scene) *per_scene = st;
+  const int errbits = ST_NONFINITE | ST_BOUNDS;
   int rc = MPM_OK;
-  for (int s = 0; s < ctx->nscenes; ++s)
-    if (st[s] && rc == MPM_OK) {
+  bool any = false;
+  for (int s = 0; s < ctx->nscenes; ++s) {
+    any = any || (all ? st[s] != 0 : (st[s] & errbits) != 0);
+    if ((st[s] & errbits) && rc == MPM_OK) {
       rc = status_code(st[s]);
       ctx->err = (st[s] & ST_NONFINITE)
-                     ? "SVD Error: non-finite deformation gradient (particle F left unchanged)"
+                     ? "non-finite state: the SVD met a non-finite F (particle F left unchanged) or a "
+                       "particle's velocity or stress input is not finite"
                      : "device status " + std::to_string(st[s]);
       if (ctx->nscenes > 1) ctx->err = "scene " + std::to_string(s) + ": " + ctx->err;
     }
-  if (rc != MPM_OK) {
-    MPM_HIP(hipMemsetAsync(ctx->status, 0, sizeof(int) * ctx->nscenes, ctx->stream));
+  }
+  if (any) {
+    if (all) {
+      MPM_HIP(hipMemsetAsync(ctx->status, 0, sizeof(int) * ctx->nscenes, ctx->stream));
+    } else {
+      // clear the error bits, keep pending warnings
+      for (int s = 0; s < ctx->nscenes; ++s) st[s] &= ~errbits;
+      MPM_HIP(hipMemcpyAsync(ctx->status, st.data(), sizeof(int) * ctx->nscenes, hipMemcpyHostToDevice,
+                             ctx->stream));
+    }
     MPM_HIP(hipStreamSynchronize(ctx->stream));
   }
   return rc;
@@ -868,7 +886,7 @@ int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms) {
   const int rrc = restore();
   if (!rc) rc = rrc;
   if (!rc) MPM_HIP(hipStreamSynchronize(ctx->stream));
-  const int src = read_status(ctx);  // flags of the candidate runs, cleared
+  const int src = read_status(ctx, nullptr, true);  // flags of the candidate runs, cleared
   if (!rc && src) {
     rc = src;
     ctx->err = "mpm_autotune: a candidate run raised a status flag (" + ctx->err +
@@ -946,7 +964,7 @@ int mpm_scene_status(mpm_ctx* ctx, int32_t* status) {
   if (check_ctx(ctx) || !status) return MPM_ERR_ARG;
   MPM_HIP(hipStreamSynchronize(ctx->stream));
   std::vector<int> st;
-  const int rc = read_status(ctx, &st);
+  const int rc = read_status(ctx, &st, true);
   for (int s = 0; s < ctx->nscenes; ++s) status[s] = status_code(st[s]);
   return rc == MPM_ERR_HIP ? rc : MPM_OK;
 }

@@ -39,6 +39,7 @@ using blk::PLANE;
 using blk::claim_block;
 
 enum Mode { MODE_P2G = 0, MODE_FUSED = 1, MODE_G2P = 2, MODE_MASS = 3 };
+static_assert(ST_NONFINITE == 1 && ST_CFL == 4, "stability_flags returns the status bits");
 
 // Field f of slot p: a wave-uniform row base plus a 32-bit byte offset, which
 // maps to one scalar-base global access per field (slots < 2^30).
@@ -328,7 +329,9 @@ __device__ __forceinline__ void p2g_one(const BlockArgs& ba, const Scene& S, int
     const float V0 = ld(a, FVOL, p);
     const float E = ld(a, FE0, p);
     const M3 Dinv = dinv_matrix(S.dinv);
-    const M3 A = stress_matrix(E, S.lame, V0, Dinv, FE, FP);
+    float stiff;
+    const M3 A = stress_matrix(E, S.lame, V0, Dinv, FE, FP, stiff);
+    if (const int fl = stability_flags(v, stiff, V0, m, a.dt, S.h)) atomicOr(a.status + scene, fl);
     const M3 BD = mmul(B, Dinv);
 #pragma unroll
     for (int c = 0; c < 3; ++c)

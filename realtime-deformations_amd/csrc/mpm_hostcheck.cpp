@@ -79,6 +79,22 @@ int hc_fupdate(int compose, float dt, float h, double theta_c, double theta_s, c
   return ok ? 0 : 1;
 }
 
+// hc_fupdate with the TEXTBOOK FP64 re-solve threshold (svd_gap_fp64)
+int hc_fupdate_gap(int compose, float dt, float h, double theta_c, double theta_s, float gap64, const float* B,
+                   const float* FE, const float* FP, float* FEo, float* FPo) {
+  M3 fe = load9(FE), fp = load9(FP);
+  const bool ok = f_update(compose, dt, dp_inverse(h), (float)(1.0 - theta_c), (float)(1.0 + theta_s), gap64,
+                           load9(B), fe, fp);
+  store9(fe, FEo);
+  store9(fp, FPo);
+  return ok ? 0 : 1;
+}
+
+// The fused pass's per-particle stability check (0, 1 non-finite, 4 CFL).
+int hc_stability_flags(const float* v, float stiff, float V0, float m, float dt, float h) {
+  return stability_flags(f3(v[0], v[1], v[2]), stiff, V0, m, dt, h);
+}
+
 // The byte ranges mpm_get_positions hands to nt copy threads for a read of
 // `bytes` bytes: begin[t], end[t] for t < nt.
 void hc_readback_parts(size_t bytes, unsigned nt, size_t* begin, size_t* end) {

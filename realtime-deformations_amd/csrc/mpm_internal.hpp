@@ -29,7 +29,10 @@ enum Field : int {
   NFIELDS = 36
 };
 
-enum StatusBits : int { ST_NONFINITE = 1, ST_BOUNDS = 2 };
+// Device status bits per scene.  ST_CFL (a particle moved more than a cell,
+// or the elastic wave crosses more than a cell, in one substep) is a
+// warning: only mpm_scene_status reports it.
+enum StatusBits : int { ST_NONFINITE = 1, ST_BOUNDS = 2, ST_CFL = 4 };
 
 // A device allocation freed when the scope ends, error returns included.
 template <typename T> struct DevBuf {

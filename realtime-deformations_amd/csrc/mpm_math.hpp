@@ -534,7 +534,10 @@ MPM_HD bool f_update(int compose, float dt, const M3& Dinv, float lo, float hi, 
 struct Lame {
   float nu, xi, dmu, rmu, dlam, rlam;
 };
-MPM_HD M3 stress_matrix(float E, const Lame& L, float V0, const M3& Dinv, const M3& FE, const M3& FP) {
+// `stiff` receives the hardened P-wave modulus lambda + 2 mu, which sets
+// the elastic wave speed c^2 = (lambda + 2 mu) / rho of the stability check.
+MPM_HD M3 stress_matrix(float E, const Lame& L, float V0, const M3& Dinv, const M3& FE, const M3& FP,
+                        float& stiff) {
   const float nu = L.nu, xi = L.xi;
   const float mu0 = div_rcp(E, L.dmu, L.rmu);  // E / (2 (1 + nu))
   const float jp = mdet(FP);
@@ -542,11 +545,31 @@ MPM_HD M3 stress_matrix(float E, const Lame& L, float V0, const M3& Dinv, const 
   const float mu = mu0 * hard;
   const float lambda0 = div_rcp(E * nu, L.dlam, L.rlam);  // E nu / ((1 + nu)(1 - 2 nu))
   const float lambda = lambda0 * hard;
+  stiff = fmaf(2.0f, mu, lambda);
   const M3 R = polar_R(FE);
   const float J = mdet(FE);
   const M3 FT = mtrans(minverse(FE));
   const M3 P = madd(mscale(msub(FE, R), 2.0f * mu), mscale(FT, lambda * (J - 1.0f) * J));
   return mmul(mmul(mscale(Dinv, V0), P), mtrans(FE));
+}
+MPM_HD M3 stress_matrix(float E, const Lame& L, float V0, const M3& Dinv, const M3& FE, const M3& FP) {
+  float stiff;
+  return stress_matrix(E, L, V0, Dinv, FE, FP, stiff);
+}
+
+// Explicit-integration stability of one particle for one substep:
+// |v| dt <= h (it moves at most one cell) and c dt <= h with
+// c^2 = (lambda + 2 mu) V0 / m (the elastic wave crosses at most one cell).
+// Returns 0 when both hold, ST-style bits otherwise: 1 when a quantity is not
+// finite, 4 when the bound is exceeded by finite values.
+MPM_HD int stability_flags(F3 v, float stiff, float V0, float m, float dt, float h) {
+  const float dt2 = dt * dt, h2 = h * h;
+  const float vv = v.x * v.x + v.y * v.y + v.z * v.z;
+  const float cc = stiff * V0;
+  if (vv * dt2 <= h2 && cc * dt2 <= m * h2) return 0;
+  // x - x is 0 exactly for finite x, NaN for NaN and infinities
+  const float z = (v.x - v.x) + (v.y - v.y) + (v.z - v.z) + (stiff - stiff) + (V0 - V0) + (m - m);
+  return z == 0.0f ? 4 : 1;
 }
 
 // ---------------------------------------------------------------- colliders

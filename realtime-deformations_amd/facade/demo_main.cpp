@@ -5,9 +5,11 @@
 //
 // writes OUT_PREFIX.init.f32 (x, v, m of every particle after
 // initializeParticles), OUT_PREFIX.colliders.f32 (13 floats per collider:
-// translation, quaternion w x y z, scale, velocity) and OUT_PREFIX.final.f32
-// (x, v, F_E per particle after FRAMES substeps), and prints per-stage wall
-// times like main.cpp's timers.
+// translation, quaternion w x y z, scale, velocity), OUT_PREFIX.final.f32
+// (x, v, F_E per particle after FRAMES substeps) and OUT_PREFIX.drawn.f32
+// (x, y, z, size per particle as the viewer read them in the last frame), and
+// prints the time per frame and the number of device reads of the particle
+// state that the per-frame getParticles() loop made.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -104,6 +106,8 @@ int main(int argc, char** argv) {
   }
 
   const double delta = 1e-5;
+  const int64_t reads0 = MPM.particleReads();
+  std::vector<float> drawn(4 * (size_t)numParticles);
   const auto t0 = std::chrono::steady_clock::now();
   for (int f = 0; f < frames; ++f) {
     for (auto& box : solidObjects) box.move((ftype)delta);
@@ -114,7 +118,16 @@ int main(int argc, char** argv) {
     MPM.updateDeformationGradient((ftype)delta);
     MPM.updateParticleVelocities();
     MPM.updateParticlePositions((ftype)delta);
+    // the viewer's read (main.cpp:257-271): getParticles() once per particle
+    for (int i = 0; i < MPM.getNumParticles(); i++) {
+      const auto q = MPM.getParticles()[i];
+      drawn[4 * i + 0] = q.pos.x;
+      drawn[4 * i + 1] = q.pos.y;
+      drawn[4 * i + 2] = q.pos.z;
+      drawn[4 * i + 3] = q.size;
+    }
   }
+  const int64_t frame_reads = MPM.particleReads() - reads0;
   const Particle* p = MPM.getParticles();
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   std::vector<float> fin;
@@ -132,7 +145,10 @@ int main(int argc, char** argv) {
   // the viewer's positions-only read (pinned mirror)
   const float* pos = MPM.getPositions();
   dump(prefix + ".positions.f32", std::vector<float>(pos, pos + 3 * (size_t)numParticles));
+  dump(prefix + ".drawn.f32", drawn);
   std::printf("frames %d  ms/frame %.4f  centre %.6f %.6f %.6f\n", frames, ms / frames, cx / numParticles,
               cy / numParticles, cz / numParticles);
+  std::printf("particle reads %lld in %d frames of %d getParticles() calls each\n", (long long)frame_reads, frames,
+              numParticles);
   return 0;
 }

@@ -250,6 +250,7 @@ void LagrangeEulerView::initializeParticles(const v3t& particlesOrigin, const v3
     m[i] = particles[i].mass;
   }
   check(mpm_set_particles(ctx, x.data(), v.data(), m.data(), nullptr), "initializeParticles");
+  mirrorStale = false;  // device state = the mirror just placed
 }
 
 void LagrangeEulerView::commitParticles() {
@@ -273,6 +274,7 @@ void LagrangeEulerView::commitParticles() {
   }
   check(mpm_set_particles(ctx, x.data(), v.data(), m.data(), nullptr), "commitParticles");
   check(mpm_set_state(ctx, B.data(), FE.data(), FP.data(), V0.data()), "commitParticles");
+  mirrorStale = false;  // device state = the mirror just uploaded
 }
 
 void LagrangeEulerView::download() {
@@ -295,9 +297,17 @@ void LagrangeEulerView::download() {
   }
 }
 
+// The reference returns a pointer into its live particle array
+// (material_point_method.hpp:181-183) and its viewer calls this once per
+// particle (main.cpp:257-271); the mirror is re-read only when the device
+// state changed since the last read.
 Particle* LagrangeEulerView::getParticles() {
   flush();
-  download();
+  if (mirrorStale) {
+    download();
+    ++reads;
+    mirrorStale = false;
+  }
   return particles.data();
 }
 
@@ -329,6 +339,7 @@ void LagrangeEulerView::run_stage(int stage, ftype dt) {
   static const int map[] = {MPM_STAGE_P2G, MPM_STAGE_FORCES, MPM_STAGE_GRID, MPM_STAGE_COLLIDE,
                             MPM_STAGE_FUPDATE, MPM_STAGE_G2P, MPM_STAGE_ADVECT};
   check(mpm_stage(ctx, map[stage], dt), "stage");
+  mirrorStale = true;
 }
 
 // The canonical order of main.cpp:192-218; a completed run with one dt
@@ -346,6 +357,7 @@ void LagrangeEulerView::enqueue(int stage, ftype dt) {
     const ftype d = queue[P_GRID].dt;
     if (queue[P_COLLIDE].dt == d && queue[P_FUPDATE].dt == d && queue[P_ADVECT].dt == d) {
       check(mpm_substeps(ctx, d, 1), "substep");
+      mirrorStale = true;
       queue.clear();
     } else {
       flush();
@@ -372,6 +384,7 @@ void LagrangeEulerView::rasterizeParticlesToGrid() { enqueue(P_P2G, 0.0f); }
 void LagrangeEulerView::computeParticleVolumesAndDensities() {
   flush();
   check(mpm_stage(ctx, MPM_STAGE_VOLUMES, 0.0f), "computeParticleVolumesAndDensities");
+  mirrorStale = true;
 }
 void LagrangeEulerView::computeExplicitGridForces() { enqueue(P_FORCES, 0.0f); }
 void LagrangeEulerView::gridVelocitiesUpdate(ftype dt) { enqueue(P_GRID, dt); }
@@ -387,6 +400,7 @@ void LagrangeEulerView::substeps(ftype dt, int n, const std::vector<MeshCollider
   flush();
   set_colliders(objects);
   check(mpm_substeps(ctx, dt, n), "substeps");
+  mirrorStale = true;
 }
 
 }  // namespace MaterialPointMethod

@@ -17,8 +17,9 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libmpm_hip.so")
 
 MPM_OK = 0
 MPM_ERR_ARG, MPM_ERR_HIP, MPM_ERR_BOUNDS, MPM_ERR_NONFINITE, MPM_ERR_STATE, MPM_ERR_NOMEM = 1, 2, 3, 4, 5, 6
+MPM_ERR_CFL = 7  # warning: reported by scene_status only
 ERRORS = {1: "MPM_ERR_ARG", 2: "MPM_ERR_HIP", 3: "MPM_ERR_BOUNDS", 4: "MPM_ERR_NONFINITE", 5: "MPM_ERR_STATE",
-          6: "MPM_ERR_NOMEM"}
+          6: "MPM_ERR_NOMEM", 7: "MPM_ERR_CFL"}
 COMPOSE_LITERAL, COMPOSE_TEXTBOOK = 0, 1
 BC_COLLIDERS, BC_DOMAIN = 1, 2
 STAGE_P2G, STAGE_VOLUMES, STAGE_FORCES, STAGE_GRID, STAGE_COLLIDE, STAGE_FUPDATE, STAGE_G2P, STAGE_ADVECT = range(8)
@@ -396,7 +397,9 @@ class Simulation:
         return out
 
     def scene_status(self):
-        """Per-scene MPM_OK / MPM_ERR_* since the last status read."""
+        """Per-scene MPM_OK / MPM_ERR_* since the last status read
+        (MPM_ERR_NONFINITE, then MPM_ERR_BOUNDS, then the MPM_ERR_CFL
+        warning); clears every flag."""
         st = (C.c_int32 * self.nscenes)()
         self._check(self.L.mpm_scene_status(self.ctx, st))
         return list(st)

@@ -118,6 +118,24 @@ def test_nonfinite_scene_is_flagged_alone():
     b.close()
 
 
+def test_cfl_warning_for_stiff_and_fast_scenes():
+    """Per-substep stability flags on the device: a scene of E0 = 5e7 at 128^3
+    and dt = 1e-4 (elastic wave dt c / h ~ 4.7) and a scene at 150 m/s
+    (|v| dt / h ~ 1.9) raise the CFL warning for their own scene only; a
+    scene of the stiffest configured modulus (E0 = 5e5 at 128^3, ~0.47)
+    does not.  The warning does not fail synchronising calls; scene_status
+    reports it once and clears it."""
+    dt = 1e-4
+    ok = scenes.snowball(seed=5, n=4000, grid=128, radius=0.04, v0=(0.0, -3.0, 0.0), E=5e5)
+    stiff = scenes.snowball(seed=6, n=4000, grid=128, radius=0.04, v0=(0.0, -1.0, 0.0), E=5e7)
+    fast = scenes.snowball(seed=7, n=4000, grid=128, radius=0.04, centre=(0.3, 0.5, 0.5), v0=(150.0, 0.0, 0.0))
+    b, got = _run_batch([ok, stiff, fast], dt, 2)
+    assert b.scene_status() == [0, M.MPM_ERR_CFL, M.MPM_ERR_CFL]
+    assert b.scene_status() == [0, 0, 0]
+    assert all(np.isfinite(g["x"]).all() for g in got)
+    b.close()
+
+
 def test_autotune_keeps_state_and_picks_a_candidate():
     scs = _mixed_batch()
     b = M.Simulation([_params(s) for s in scs], [s["x"].shape[0] for s in scs])

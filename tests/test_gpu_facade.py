@@ -39,3 +39,9 @@ def test_facade_demo_loop_matches_oracle(tmp_path):
     assert rel_err(fin[:, 0:3], ref["x"]) < 1e-5
     assert rel_err(fin[:, 3:6], ref["v"]) < 1e-4
     assert rel_err(fin[:, 6:15], ref["FE"]) < 1e-4
+    # the viewer loop calls getParticles() once per particle every frame
+    # (main.cpp:257-271): one device read per frame, not one per call
+    assert f"particle reads {frames} in {frames} frames of {n} getParticles() calls each" in out.stdout, out.stdout
+    drawn = np.fromfile(prefix + ".drawn.f32", np.float32).reshape(-1, 4)
+    np.testing.assert_array_equal(drawn[:, 0:3], fin[:, 0:3])
+    assert np.all(drawn[:, 3] == np.float32(0.02))

@@ -10,7 +10,7 @@ import pytest
 
 import mpm_amd as M
 import oracle as O
-from helpers import eye9, max_abs, rel_err, to_math, warm_state
+from helpers import eye9, fp64_textbook_update, max_abs, near_tied_spectra, rel_err, to_math, warm_state
 from mpm_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -58,6 +58,55 @@ def _compare_particles(gp, op, tol, h=None, fields=("x", "v", "B", "FE", "FP", "
     bad = {k: e for k, e in errs.items() if e > tol}
     assert not bad, f"fields over {tol}: {bad} (all: {errs})"
     return errs
+
+
+def test_textbook_fp64_resolve_on_the_device():
+    """svd_gap_fp64 > 0 (TEXTBOOK): particles whose singular values nearly tie
+    re-solve the SVD and clamp in FP64 on the device.  On near-tied pairs
+    that straddle a clamp bound the device F_E is within one FP32 ulp of the
+    oracle's FP64 update; with the re-solve off it is not."""
+    rng = np.random.default_rng(21)
+    n = 4096
+    B, FE, FP = near_tied_spectra(rng, n)
+    ref = fp64_textbook_update(B, FE, FP)
+    x = rng.uniform(0.3, 0.7, (n, 3)).astype(np.float32)
+    errs = {}
+    for gap in (0.0, 1e-3):
+        pg = M.default_params((64, 64, 64), 1 / 64, compose=M.COMPOSE_TEXTBOOK, theta_s=7.5e-3, svd_gap_fp64=gap)
+        g = M.Simulation(pg, n)
+        g.set_particles(x, np.zeros_like(x), np.ones(n, np.float32))
+        g.set_state(B=B, FE=FE, FP=FP)
+        g.stage(M.STAGE_FUPDATE, 1e-4)
+        out = g.particles()
+        g.close()
+        errs[gap] = float(np.abs(out["FE"] - ref["FE"]).max())
+        if gap:
+            np.testing.assert_allclose(out["FP"], ref["FP"], rtol=0, atol=4e-7)
+    assert errs[1e-3] < 1.2e-7, errs
+    assert errs[0.0] > 4 * errs[1e-3], errs
+
+
+def test_fused_textbook_with_fp64_resolve_matches_fp64_oracle():
+    """The fused path with the FP64 re-solve on (TEXTBOOK, svd_gap_fp64 =
+    1e-3) on a warm two-snowball state: 1 and 10 substeps against the
+    oracle's FP64 build within the FP32 tolerances."""
+    rng = np.random.default_rng(22)
+    sc = scenes.two_snowballs(n_total=16384, grid=64, dt=1e-4)
+    n = sc["x"].shape[0]
+    Bw, FEw, FPw = warm_state(rng, n, bscale=0.01, scale=5e-3)
+    g, _ = _pair(sc, compose=1, gap64=1e-3)
+    g.set_state(B=Bw, FE=FEw, FP=FPw)
+    prm = sc["params"]
+    p64 = O.make_params(sc["dims"], sc["h"], compose=1, bc_mode=prm["bc_mode"], theta_s=prm["theta_s"], threads=8)
+    o = O.Sim(p64, n, double=True)
+    o.set_particles(sc["x"], sc["v"], sc["m"], sc["E0"], B=Bw, FE=FEw, FP=FPw)
+    _init(g, o)
+    for steps, tol in ((1, 1e-5), (9, 1e-4)):
+        g.substeps(1e-4, steps)
+        g.sync()
+        o.substeps(1e-4, steps)
+        _compare_particles(g.particles(), o.particles(), tol, h=sc["h"])
+    g.close()
 
 
 def test_fupdate_bit_exact_literal_and_textbook():
