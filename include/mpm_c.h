@@ -245,7 +245,8 @@ int mpm_create_from_snapshot(const char* path, mpm_ctx** out, int64_t* step);
  * block_grid in {1024, 2048, 4096}, capped at the block count) over at least
  * `steps` substeps, rounded up to whole re-bin cycles and at least 128, from
  * the current state; it restores that state, keeps the fastest and returns
- * its time per substep. */
+ * its time per substep.  After either call the next substep re-bins, so a
+ * run of whole re-bin cycles starts there. */
 int mpm_set_tuning(mpm_ctx* ctx, int32_t resort_every, int32_t block_grid);
 int mpm_get_tuning(mpm_ctx* ctx, int32_t* resort_every, int32_t* block_grid);
 int mpm_autotune(mpm_ctx* ctx, float dt, int32_t steps, double* best_ms_per_substep);

@@ -780,11 +780,26 @@ template <typename T> struct Sim {
     }
   }
 
-  // a7: material_point_method.cpp:306-330
+  // a7: material_point_method.cpp:306-330.  The reference leaves the loop
+  // at the first particle whose SVD fails (313-316), so particles after it
+  // keep their old F.  With threads, every particle is updated into a copy
+  // in parallel and only the particles before the first failure commit.
   int deformation_update(T dt) {
+    if (prm.threads <= 1) {
+      for (int p = 0; p < n; ++p)
+        if (f_update(prm.compose, dt, Dinv, lo, hi, B[p], FE[p], FP[p])) return last_error = 3;
+      return 0;
+    }
+    std::vector<M3<T>> fe(FE), fp(FP);
+    int first_bad = n;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(prm.threads) reduction(min : first_bad)
+#endif
     for (int p = 0; p < n; ++p)
-      if (f_update(prm.compose, dt, Dinv, lo, hi, B[p], FE[p], FP[p])) return last_error = 3;
-    return 0;
+      if (f_update(prm.compose, dt, Dinv, lo, hi, B[p], fe[p], fp[p])) first_bad = std::min(first_bad, p);
+    std::copy(fe.begin(), fe.begin() + first_bad, FE.begin());
+    std::copy(fp.begin(), fp.begin() + first_bad, FP.begin());
+    return first_bad < n ? (last_error = 3) : 0;
   }
 
   // a8: material_point_method.cpp:332-342

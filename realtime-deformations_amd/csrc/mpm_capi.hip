@@ -803,6 +803,7 @@ int mpm_set_tuning(mpm_ctx* ctx, int32_t resort_every, int32_t block_grid) {
   if (check_ctx(ctx) || resort_every < 1 || block_grid < 1) return MPM_ERR_ARG;
   ctx->resort_every = resort_every;
   ctx->block_grid = block_grid;
+  ctx->since_sort = -1;  // a new re-bin cycle starts at the next substep
   return MPM_OK;
 }
 

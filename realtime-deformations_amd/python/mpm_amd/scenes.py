@@ -94,6 +94,30 @@ def slab(seed=2, n_total=4194304, grid=256, cells=(128, 32, 128), bottom=0.5, pp
                 dims=(grid, grid, grid), h=h, dt=dt, steps=steps, params=material())
 
 
+def landing_slab(seed=2, n_total=4194304, grid=256, speed=2.0, dt=2.5e-5, steps=200):
+    """The 4,194,304-particle slab of slab() moved down onto the sticky floor:
+    its bottom cell row starts at node row 3, so the floor band (node rows
+    j < 3) is inside the bottom particles' stencils from the first substep,
+    and it falls at `speed` m/s.  The impact stops the bottom layers and sends
+    a compression wave up the slab (about 24 m/s, 100 substeps for 15 cells):
+    F_E leaves the identity, the SVD rotates and the clamp turns strain into
+    plastic flow, the deforming counterpart of the free-falling slab."""
+    out = slab(seed=seed, n_total=n_total, grid=grid, bottom=3.0 / grid, dt=dt, steps=steps)
+    out["v"][:, 1] = -float(speed)
+    return out
+
+
+def compressing(sc, rate, centre=None):
+    """Adds a uniform compression rate * (centre - x) (1/s) to a scene's
+    velocities (centre: the particles' mean by default), so the body
+    deforms from the first substep: stress, hardening and the clamp act."""
+    x = sc["x"].astype(np.float64)
+    c = x.mean(0) if centre is None else np.asarray(centre, np.float64)
+    sc = dict(sc)
+    sc["v"] = (sc["v"].astype(np.float64) + rate * (c - x)).astype(np.float32)
+    return sc
+
+
 def reference_scene(seed=0, n=2147, grid=20, h=0.05, origin=(0.5, 0.6, 0.5), velocity=(0.0, -200.0, 0.0),
                     width=0.2, mass=6e-5):
     """The reference demo's placement (material_point_method.cpp:18-63):
@@ -160,3 +184,30 @@ def scene_batch(seed=3, nscenes=64, n_range=(4096, 262144), e_range=(5e4, 5e5), 
         out.append(dict(x=x, v=np.tile(v0, (n, 1)), m=np.full(n, mp, np.float32), E0=np.full(n, E, np.float32),
                         dims=(grid, grid, grid), h=h, dt=dt, steps=steps, params=material(E=E)))
     return out
+
+
+def adjoint_batch(seed=4, nscenes=16, n=32768, rate=5.0):
+    """The differentiable workload: `nscenes` snowballs of n particles on 64^3
+    (dt 1e-4, 200 substeps, uniform initial velocity in [-3, 3]^3 per scene),
+    each compressed at `rate` 1/s on top of it so stress, hardening and the
+    clamp act from the start (dL/dE0 != 0), and a seeded target point per
+    scene 0.02 from its centre; -> (scenes, targets)."""
+    scs = [compressing(sc, rate) for sc in scene_batch(seed=seed, nscenes=nscenes, n_range=(n, n), split=n,
+                                                        steps=200)]
+    rng = np.random.default_rng(seed + 100)
+    targets = [sc["x"].astype(np.float64).mean(0) + rng.normal(0.0, 0.02, 3) for sc in scs]
+    return scs, targets
+
+
+def com_loss(xs, vs, targets):
+    """Per scene |centre of mass - target|^2 + 1e-3 mean |v|^2 (uniform
+    masses: the centre of mass is the mean position), summed over scenes;
+    -> (L, [dL/dx per scene], [dL/dv per scene]) for the final particles."""
+    L, gx, gv = 0.0, [], []
+    for x, v, t in zip(xs, vs, targets):
+        x64, v64 = np.asarray(x, np.float64), np.asarray(v, np.float64)
+        d = x64.mean(0) - t
+        L += float(d @ d) + 1e-3 * float((v64 * v64).sum(1).mean())
+        gx.append(np.tile(2.0 * d / x64.shape[0], (x64.shape[0], 1)).astype(np.float32))
+        gv.append((2e-3 * v64 / x64.shape[0]).astype(np.float32))
+    return L, gx, gv

@@ -488,3 +488,34 @@ def test_g2p_recovers_an_affine_grid_field():
             for p_ in range(4):
                 np.testing.assert_allclose(C[p_], A, rtol=0, atol=tol * np.abs(A).max() * 50)
                 assert np.abs(C[p_] - A.T).max() > 0.1 * np.abs(A).max()
+
+
+def test_threaded_f_update_equals_serial_and_stops_at_first_failure():
+    """The oracle's threaded F update (used for the all-cores CPU baseline)
+    commits exactly what the serial loop does: every particle bit for bit,
+    and on an SVD failure only the particles before the first failing one
+    (material_point_method.cpp:313-316 leaves the loop there)."""
+    rng = np.random.default_rng(15)
+    n = 3000
+    B, FE, FP = warm_state(rng, n, bscale=0.05)
+    FE_bad = FE.copy()
+    FE_bad[1700, 4] = np.nan
+    x = rng.uniform(0.3, 0.7, (n, 3)).astype(np.float32)
+    for fe_in, expect_rc in ((FE, 0), (FE_bad, 1)):
+        out = {}
+        for threads in (1, 4):
+            p = O.make_params((64, 64, 64), 1 / 64, compose=O.LITERAL, theta_s=7.5e-3, threads=threads)
+            s = O.Sim(p, n)
+            s.set_particles(x, np.zeros_like(x), np.ones(n, np.float32), B=B, FE=fe_in, FP=FP)
+            if expect_rc:
+                with pytest.raises(RuntimeError):
+                    s.stage(O.STAGE_FUPDATE, 1e-4)
+            else:
+                s.stage(O.STAGE_FUPDATE, 1e-4)
+            out[threads] = s.particles()
+        np.testing.assert_array_equal(out[1]["FE"], out[4]["FE"])
+        np.testing.assert_array_equal(out[1]["FP"], out[4]["FP"])
+        if expect_rc:
+            # particles from the failing one on keep their input F
+            np.testing.assert_array_equal(out[4]["FE"][1701:], fe_in[1701:].astype(np.float64))
+            assert not np.array_equal(out[4]["FE"][:1700], fe_in[:1700].astype(np.float64))
