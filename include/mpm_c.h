@@ -42,8 +42,8 @@ enum {
   MPM_ERR_CFL = 7        /* warning, reported by mpm_scene_status only: on the
                             fused path a particle moved more than one cell in a
                             substep (|v| dt > h), or the elastic wave of its
-                            hardened moduli crosses more than one cell
-                            (dt sqrt((lambda + 2 mu) V0 / m) > h)               */
+                            hardened moduli crosses more than two cells
+                            (dt sqrt((lambda + 2 mu) V0 / m) > 2h)              */
 };
 
 /* F_E recomposition after the singular-value clamp

@@ -44,10 +44,22 @@
 //                      orthonormality, det +1, RᵀF symmetric.
 //   box_sdf / collide  the reference's SDF on glm (oracle/_ref) for SDF
 //                      values; hand-computed response cases.
-//   p2g / grid update / g2p / advect
-//                      mass and momentum conservation, uniform-velocity
-//                      closed forms (B = 0, v_p = v_i), exact free fall over
-//                      100 substeps (closed form), clamp bounds.
+//   p2g                mass and momentum conservation; affine
+//                      reproduction: one particle with C = B D^-1 = A
+//                      (non-symmetric) gives v_i = v_p + A (x_i - x_p).
+//   init_volumes       exact rationals (dyadic x / h): one isolated particle
+//                      has V0 = h^3 / prod_axes sum_d N(u_d)^2; overlapping
+//                      particles against a brute-force rational sum.
+//   grid update        sum m_i v_i = dt M g and the first moment
+//                      sum m_i v_i x_i^T = dt (-sum_p V0 P F^T +
+//                      g sum_p m_p x_p^T) against an FP64 stress.
+//   g2p                uniform velocity gives B = 0, v_p = v_i; an affine
+//                      grid field b + A x gives v_p = b + A x_p and
+//                      B D^-1 = A (non-symmetric A).
+//   advect             exact free fall over 100 substeps (closed form),
+//                      clamp bounds.
+//   threaded F update  bit-equal to the serial loop, stopping at the first
+//                      failing particle as the reference's loop does.
 //   forces             F_E = I and F_E = R give zero force; small-strain
 //                      linearisation against a finite difference of the
 //                      elastic energy.

@@ -200,8 +200,9 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
     const M3 K = mtrans(minverse(FE));  // F_E^-T
     const float cJ = (J - 1.0f) * J;
     const M3 Pk = madd(mscale(msub(FE, R), 2.0f * mu), mscale(K, lam * cJ));
-    const M3 A = mmul(mscale(Dinv, V0), mmul(Pk, mtrans(FE)));
-    const M3 BD = mmul(B, Dinv);
+    // D^-1 = d I (mul_dinv): the products with it are scalings
+    const M3 A = mscale(mmul(Pk, mtrans(FE)), Dinv.c[0][0] * V0);
+    const M3 BD = mul_dinv(B, Dinv);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -263,13 +264,13 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
   const M3 K = mtrans(minverse(FE));  // F_E^-T
   const float cJ = (J - 1.0f) * J;
   const M3 Pk = madd(mscale(msub(FE, R), 2.0f * mu), mscale(K, lam * cJ));
-  const M3 BD = mmul(B, Dinv);
+  const M3 BD = mul_dinv(B, Dinv);
   // Q = m B D - dt A
-  M3 gB = mscale(mmul(gQ, mtrans(Dinv)), m);
+  M3 gB = mscale(mul_dinv(gQ, Dinv), m);
   const M3 gA = mscale(gQ, -a.dt);
   // A = V0 D P F_E^T
-  const M3 gPk = mscale(mmul(mmul(mtrans(Dinv), gA), FE), V0);
-  M3 gFE = mmul(mtrans(gA), mscale(mmul(Dinv, Pk), V0));
+  const M3 gPk = mscale(mmul(mscale(gA, Dinv.c[0][0]), FE), V0);
+  M3 gFE = mmul(mtrans(gA), mscale(mscale(Pk, Dinv.c[0][0]), V0));
   // P = 2 mu (F - R) + lam c(J) K
   const float gmu = 2.0f * mdot(gPk, msub(FE, R));
   const float glam = cJ * mdot(gPk, K);
@@ -312,7 +313,7 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
     gFP = madd(gFP, mmul(mtrans(Fh), gFtot));
     // F^ = (I + dt B D) F_E
     gFE = madd(gFE, mmul(mtrans(madd(mdiag(1.0f), mscale(BD, a.dt))), gFh));
-    gB = madd(gB, mscale(mmul(mmul(gFh, mtrans(FE)), mtrans(Dinv)), a.dt));
+    gB = madd(gB, mscale(mul_dinv(mmul(gFh, mtrans(FE)), Dinv), a.dt));
   }
   Gout[(int64_t)AX * sd + p] = gx.x;
   Gout[(int64_t)(AX + 1) * sd + p] = gx.y;

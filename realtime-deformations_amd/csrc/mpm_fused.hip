@@ -245,6 +245,9 @@ __device__ __forceinline__ void p2g_scatter(const Scene& S, const StencilInfo& s
   }
 }
 
+__device__ __forceinline__ void scatter_global(const BlockArgs& ba, const Scene& S, const StencilInfo& s, F3 x, float m,
+                                            F3 mv, const M3& Q);
+
 // Scatter into the LDS halo, or with global atomics (and the touched grid
 // blocks marked) when the stencil left the halo.
 __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S, double* __restrict__ ah,
@@ -273,6 +276,16 @@ __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S,
       atomicAdd(&ah[3 * PLANE + q], (double)pz);
     });
   } else {
+    scatter_global(ba, S, s, x, m, mv, Q);
+  }
+}
+
+// The global-memory scatter of a particle whose stencil left its block's
+// halo: FP32 atomics into the grid accumulators, touched grid blocks marked.
+__device__ __forceinline__ void scatter_global(const BlockArgs& ba, const Scene& S, const StencilInfo& s, F3 x, float m,
+                                            F3 mv, const M3& Q) {
+  const KArgs& a = ba.a;
+  {
     // stencil blocks: base-1 .. base+2 spans at most two blocks per axis
     for (int di = 0; di < 2; ++di)
       for (int dj = 0; dj < 2; ++dj)
@@ -291,13 +304,13 @@ __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S,
   }
 }
 
-// Stress from F^n, F update F^n -> F^{n+1} with B, then the P2G scatter.
+// Stress from F^n and the F update F^n -> F^{n+1} with B of one particle;
+// returns its mass and the affine matrix of its P2G momentum,
+// Q = m B D^-1 - dt V0 D^-1 P F^T.
 template <int COMPOSE, int REC>
-__device__ __forceinline__ void p2g_one(const BlockArgs& ba, const Scene& S, int scene, double* __restrict__ ah,
-                                        const int o[3], int p, F3 x, F3 v, const M3& B) {
+__device__ __forceinline__ void particle_update(const BlockArgs& ba, const Scene& S, int scene, int p, F3 x, F3 v,
+                                                const M3& B, float& m, M3& Q) {
   const KArgs& a = ba.a;
-  M3 Q;
-  float m;
   {
     M3 FE, FP;
 #pragma unroll
@@ -332,7 +345,7 @@ __device__ __forceinline__ void p2g_one(const BlockArgs& ba, const Scene& S, int
     float stiff;
     const M3 A = stress_matrix(E, S.lame, V0, Dinv, FE, FP, stiff);
     if (const int fl = stability_flags(v, stiff, V0, m, a.dt, S.h)) atomicOr(a.status + scene, fl);
-    const M3 BD = mmul(B, Dinv);
+    const M3 BD = mul_dinv(B, Dinv);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -349,6 +362,15 @@ __device__ __forceinline__ void p2g_one(const BlockArgs& ba, const Scene& S, int
       atomicOr(a.status + scene, ST_NONFINITE);
     }
   }
+}
+
+// Stress, F update, then the per-particle P2G scatter.
+template <int COMPOSE, int REC>
+__device__ __forceinline__ void p2g_one(const BlockArgs& ba, const Scene& S, int scene, double* __restrict__ ah,
+                                        const int o[3], int p, F3 x, F3 v, const M3& B) {
+  float m;
+  M3 Q;
+  particle_update<COMPOSE, REC>(ba, S, scene, p, x, v, B, m, Q);
   scatter_one(ba, S, ah, o, x, m, v * m, Q);
 }
 

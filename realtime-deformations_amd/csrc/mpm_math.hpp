@@ -151,6 +151,13 @@ MPM_HD M3 dinv_matrix(float d) {
   return r;
 }
 
+// B D^-1 for D^-1 = dinv_matrix(d): glm's product adds B's other columns
+// times the signed off-diagonal zeros, which changes no value, only possibly
+// the sign of a zero entry, and every use adds the identity or another
+// term first; so it is B's columns scaled by d (9 products instead of 45
+// operations).  Non-finite B stays non-finite either way.
+MPM_HD M3 mul_dinv(const M3& B, const M3& Dinv) { return mscale(B, Dinv.c[0][0]); }
+
 // -------------------------------------------------------- cubic B-spline
 // N(u) of material_point_method.hpp:20-31.  The reference's double literals
 // promote the polynomial to double before the final rounding; FP64 on gfx950
@@ -487,7 +494,7 @@ template <int COMPOSE>
 MPM_HD bool f_update_t(float dt, const M3& Dinv, float lo, float hi, float gap64, const M3& Bn,
                                            M3& FE, M3& FP) {
   const int compose = COMPOSE;
-  const M3 FPn1 = mmul(mmul(madd(mdiag(1.0f), mscale(mmul(Bn, Dinv), dt)), FE), FP);
+  const M3 FPn1 = mmul(mmul(madd(mdiag(1.0f), mscale(mul_dinv(Bn, Dinv), dt)), FE), FP);
   const M3 FEhat = mmul(FPn1, minverse(FP));
   float M[3][3], U[3][3], S[3], V[3][3];
 #pragma unroll
@@ -550,7 +557,8 @@ MPM_HD M3 stress_matrix(float E, const Lame& L, float V0, const M3& Dinv, const 
   const float J = mdet(FE);
   const M3 FT = mtrans(minverse(FE));
   const M3 P = madd(mscale(msub(FE, R), 2.0f * mu), mscale(FT, lambda * (J - 1.0f) * J));
-  return mmul(mmul(mscale(Dinv, V0), P), mtrans(FE));
+  // (D^-1 V0) P: the diagonal scaling of glm's product (see mul_dinv)
+  return mmul(mscale(P, Dinv.c[0][0] * V0), mtrans(FE));
 }
 MPM_HD M3 stress_matrix(float E, const Lame& L, float V0, const M3& Dinv, const M3& FE, const M3& FP) {
   float stiff;
@@ -558,15 +566,18 @@ MPM_HD M3 stress_matrix(float E, const Lame& L, float V0, const M3& Dinv, const 
 }
 
 // Explicit-integration stability of one particle for one substep:
-// |v| dt <= h (it moves at most one cell) and c dt <= h with
-// c^2 = (lambda + 2 mu) V0 / m (the elastic wave crosses at most one cell).
+// |v| dt <= h (it moves at most one cell) and c dt <= 2h with
+// c^2 = (lambda + 2 mu) V0 / m (the elastic wave crosses at most two cells;
+// the margin covers surface particles, whose initial volumes come from a
+// grid density that their missing neighbours make low, so m / V0
+// underestimates their density by up to a few times).
 // Returns 0 when both hold, ST-style bits otherwise: 1 when a quantity is not
-// finite, 4 when the bound is exceeded by finite values.
+// finite, 4 when a bound is exceeded by finite values.
 MPM_HD int stability_flags(F3 v, float stiff, float V0, float m, float dt, float h) {
   const float dt2 = dt * dt, h2 = h * h;
   const float vv = v.x * v.x + v.y * v.y + v.z * v.z;
   const float cc = stiff * V0;
-  if (vv * dt2 <= h2 && cc * dt2 <= m * h2) return 0;
+  if (vv * dt2 <= h2 && cc * dt2 <= 4.0f * m * h2) return 0;
   // x - x is 0 exactly for finite x, NaN for NaN and infinities
   const float z = (v.x - v.x) + (v.y - v.y) + (v.z - v.z) + (stiff - stiff) + (V0 - V0) + (m - m);
   return z == 0.0f ? 4 : 1;

@@ -120,10 +120,10 @@ def test_nonfinite_scene_is_flagged_alone():
 
 def test_cfl_warning_for_stiff_and_fast_scenes():
     """Per-substep stability flags on the device: a scene of E0 = 5e7 at 128^3
-    and dt = 1e-4 (elastic wave dt c / h ~ 4.7) and a scene at 150 m/s
-    (|v| dt / h ~ 1.9) raise the CFL warning for their own scene only; a
-    scene of the stiffest configured modulus (E0 = 5e5 at 128^3, ~0.47)
-    does not.  The warning does not fail synchronising calls; scene_status
+    and dt = 1e-4 (elastic wave dt c / h ~ 4.7, over the bound of 2) and a
+    scene at 150 m/s (|v| dt / h ~ 1.9, over the bound of 1) raise the CFL
+    warning for their own scene only; a scene of the stiffest configured
+    modulus (E0 = 5e5 at 128^3, ~0.47) does not.  The warning does not fail synchronising calls; scene_status
     reports it once and clears it."""
     dt = 1e-4
     ok = scenes.snowball(seed=5, n=4000, grid=128, radius=0.04, v0=(0.0, -3.0, 0.0), E=5e5)

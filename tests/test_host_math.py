@@ -163,8 +163,8 @@ def test_positions_readback_parts_cover_every_byte(hc):
 def test_stability_flags_cfl_and_nonfinite(hc):
     """The fused pass's per-particle check, host build of the same source:
     a particle may move at most one cell per substep (|v| dt <= h) and the
-    elastic P-wave of its hardened moduli may cross at most one cell
-    (dt sqrt((lambda + 2 mu) V0 / m) <= h).  Exceeding either with finite
+    elastic P-wave of its hardened moduli may cross at most two cells
+    (dt sqrt((lambda + 2 mu) V0 / m) <= 2h).  Exceeding either with finite
     values is the CFL warning (4); any non-finite input is 1."""
     hc.hc_stability_flags.argtypes = [_F, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]
     h, dt = 1 / 128, 1e-4
@@ -178,8 +178,8 @@ def test_stability_flags_cfl_and_nonfinite(hc):
     assert flags([0.99 * vmax, 0, 0], 1e5) == 0
     assert flags([1.01 * vmax, 0, 0], 1e5) == 4
     assert flags([0.6 * vmax, 0.6 * vmax, 0.6 * vmax], 1e5) == 4  # |v| counts, not components
-    # stiffness with c dt = h: (lambda + 2 mu) = rho (h / dt)^2
-    smax = 400.0 * vmax ** 2
+    # stiffness with c dt = 2h: (lambda + 2 mu) = rho (2 h / dt)^2
+    smax = 400.0 * (2 * vmax) ** 2
     assert flags([0, 0, 0], 0.99 * smax) == 0
     assert flags([0, 0, 0], 1.01 * smax) == 4
     # configs' stiffest batch scene: E0 = 5e5 hardened by exp(10 * 0.05) at 128^3
