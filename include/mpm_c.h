@@ -214,6 +214,20 @@ typedef struct mpm_generator {
 int mpm_scene_generate(mpm_ctx* ctx, int32_t scene, const mpm_generator* g, int64_t* candidates);
 int mpm_generate(mpm_ctx* ctx, const mpm_generator* g, int64_t* candidates);
 
+/* ---- readback that allocates nothing (no counterpart in the reference).
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The same
+ * fields as mpm_scene_get_particles (caller order, NULL skips), equal to them
+ * bit for bit, without any device allocation: each field is staged through a
+ * file-scope device array of 9 * MPM_NOALLOC_MAX_PARTICLES floats (9 MiB),
+ * which every GPU the library runs on holds until the process ends.  A scene
+ * with more particles than MPM_NOALLOC_MAX_PARTICLES is MPM_ERR_ARG.  Calls
+ * on one GPU share that array, each leaving its last field there: calls that
+ * overlap on one GPU (from several threads or contexts) race. */
+#define MPM_NOALLOC_MAX_PARTICLES 262144
+int mpm_scene_get_particles_noalloc(mpm_ctx* ctx, int32_t scene, float* x, float* v, float* B, float* FE,
+                                    float* FP, float* V0);
+
 /* ---- readback and snapshots ---------------------------------------------
  * m and E0 of one scene (caller order; NULL skips). */
 int mpm_scene_get_material(mpm_ctx* ctx, int32_t scene, float* m, float* E0);

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <string>
 #include <vector>
@@ -192,9 +193,70 @@ int ensure_readback(mpm_ctx* ctx) {
   return MPM_OK;
 }
 
+// ---- readback that allocates nothing: a file-scope staging array.
+// Each field of one scene is gathered into caller order in `scratch` and
+// copied to the caller's host array.  In host code `scratch` names a
+// host-side stand-in, not the device array: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+constexpr int64_t NOALLOC_SCRATCH_FLOATS = 9 * (int64_t)MPM_NOALLOC_MAX_PARTICLES;
+static __device__ __align__(256) float scratch[NOALLOC_SCRATCH_FLOATS];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+float* scratch_dev[NOALLOC_MAX_DEVICES] = {};  // device address of `scratch`, per device
+
+int scratch_address(mpm_ctx* ctx, float** out) {
+  int dev = -1;
+  IO_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    ctx->err = "mpm_scene_get_particles_noalloc: device index out of the supported range";
+    return MPM_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    IO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    scratch_dev[dev] = static_cast<float*>(p);
+  }
+  *out = scratch_dev[dev];
+  return MPM_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int mpm_scene_get_particles_noalloc(mpm_ctx* ctx, int32_t scene, float* x, float* v, float* B, float* FE,
+                                    float* FP, float* V0) {
+  if (!ctx) return MPM_ERR_ARG;
+  if (scene < 0 || scene >= ctx->nscenes) {
+    ctx->err = "scene index out of range";
+    return MPM_ERR_ARG;
+  }
+  const Scene& S = ctx->scenes[scene];
+  if (S.np > MPM_NOALLOC_MAX_PARTICLES) {
+    ctx->err = "mpm_scene_get_particles_noalloc: the scene has " + std::to_string(S.np) +
+               " particles, more than the staging array holds (MPM_NOALLOC_MAX_PARTICLES = " +
+               std::to_string(MPM_NOALLOC_MAX_PARTICLES) + ")";
+    return MPM_ERR_ARG;
+  }
+  float* stage = nullptr;
+  if (int rc = scratch_address(ctx, &stage)) return rc;
+  if (S.np == 0) return MPM_OK;
+  const Block fields[6] = {{FX, 3}, {FV, 3}, {FB, 9}, {FFE, 9}, {FFP, 9}, {FVOL, 1}};
+  float* const outs[6] = {x, v, B, FE, FP, V0};
+  for (int f = 0; f < 6; ++f) {
+    if (!outs[f]) continue;
+    // every element [0, np * comps) of the stage is written: caller indices
+    // of a scene's slots are a permutation of [p0, p0 + np)
+    launch_gather_field(stage, ctx->P + (int64_t)fields[f].field * ctx->stride, ctx->stride, ctx->pid, S.p0, S.np,
+                        fields[f].comps, ctx->stream);
+    IO_HIP(hipGetLastError());
+    IO_HIP(hipMemcpyAsync(outs[f], stage, sizeof(float) * (size_t)S.np * fields[f].comps, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    IO_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return MPM_OK;
+}
 
 int mpm_scene_get_material(mpm_ctx* ctx, int32_t scene, float* m, float* E0) {
   if (!ctx) return MPM_ERR_ARG;

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(PKG_DIR, "lib", "libmpm_hip.so")
 MPM_OK = 0
 MPM_ERR_ARG, MPM_ERR_HIP, MPM_ERR_BOUNDS, MPM_ERR_NONFINITE, MPM_ERR_STATE, MPM_ERR_NOMEM = 1, 2, 3, 4, 5, 6
 MPM_ERR_CFL = 7  # warning: reported by scene_status only
+NOALLOC_MAX_PARTICLES = 262144  # MPM_NOALLOC_MAX_PARTICLES
 ERRORS = {1: "MPM_ERR_ARG", 2: "MPM_ERR_HIP", 3: "MPM_ERR_BOUNDS", 4: "MPM_ERR_NONFINITE", 5: "MPM_ERR_STATE",
           6: "MPM_ERR_NOMEM", 7: "MPM_ERR_CFL"}
 COMPOSE_LITERAL, COMPOSE_TEXTBOOK = 0, 1
@@ -93,6 +94,7 @@ EXPORTS = [
     "mpm_set_tuning", "mpm_get_tuning", "mpm_autotune", "mpm_adjoint_forward", "mpm_adjoint_interval", "mpm_adjoint_backward",
     "mpm_scene_generate", "mpm_generate", "mpm_scene_get_material", "mpm_get_positions", "mpm_positions_begin",
     "mpm_positions_wait", "mpm_snapshot_save", "mpm_snapshot_load", "mpm_create_from_snapshot",
+    "mpm_scene_get_particles_noalloc",
 ]
 KERNELS = {"p2g": 0, "grid": 1, "g2p": 2, "bin": 3, "fused": 4}
 NKERNELS = 8
@@ -129,6 +131,7 @@ def lib():
         L.mpm_scene_set_state.argtypes = [vp, C.c_int32, _F, _F, _F, _F]
         L.mpm_scene_set_colliders.argtypes = [vp, C.c_int32, C.POINTER(Collider), C.c_int32]
         L.mpm_scene_get_particles.argtypes = [vp, C.c_int32] + [_F] * 6
+        L.mpm_scene_get_particles_noalloc.argtypes = [vp, C.c_int32] + [_F] * 6
         L.mpm_scene_get_grid.argtypes = [vp, C.c_int32, _F, _F, _F, C.POINTER(C.c_uint8)]
         L.mpm_scene_status.argtypes = [vp, C.POINTER(C.c_int32)]
         L.mpm_set_tuning.argtypes = [vp, C.c_int32, C.c_int32]
