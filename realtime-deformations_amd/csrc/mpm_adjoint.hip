@@ -148,9 +148,12 @@ __global__ void __launch_bounds__(TPB) k_adj_grid(KArgs a, GridRec r, float4* gn
       }
     }
     const F3 gvp = f3(gn.y * mk[0], gn.z * mk[1], gn.w * mk[2]);
-    const float inv_m = 1.0f / acc.x;
-    const F3 gP = gvp * inv_m;
-    const float gm = -(acc.y * gvp.x + acc.z * gvp.y + acc.w * gvp.z) * inv_m * inv_m;
+    // dL/dP = gv' / m and dL/dm = -(P / m) . dL/dP, by division: a node
+    // reached only by stencil weights near the edge can have a denormal m,
+    // whose reciprocal overflows (inf, and 0 * inf = NaN) although both
+    // quotients are finite
+    const F3 gP = f3(gvp.x / acc.x, gvp.y / acc.x, gvp.z / acc.x);
+    const float gm = -((acc.y / acc.x) * gP.x + (acc.z / acc.x) * gP.y + (acc.w / acc.x) * gP.z);
     gnode[g] = make_float4(gm, gP.x, gP.y, gP.z);
   }
 }

@@ -186,15 +186,16 @@ def scene_batch(seed=3, nscenes=64, n_range=(4096, 262144), e_range=(5e4, 5e5), 
     return out
 
 
-def adjoint_batch(seed=4, nscenes=16, n=32768, rate=1.0):
+def adjoint_batch(seed=4, nscenes=16, n=32768, rate=1.2):
     """The differentiable workload: `nscenes` snowballs of n particles on 64^3
     (dt 1e-4, 200 substeps, uniform initial velocity in [-3, 3]^3 per scene),
     each compressed at `rate` 1/s on top of it so the stress acts from the
     start (dL/dE0 != 0), and a seeded target point per scene 0.02 from its
-    centre; -> (scenes, targets).  At 1/s the singular values of F_E stay
-    within [0.99, 1.006] over the 200 substeps, inside the clamp band: the
-    loss is smooth (no plastic-clamp kinks), where finite differences check
-    the gradients."""
+    centre; -> (scenes, targets).  At 1.2/s the singular values of F_E stay
+    within [0.9757, 0.9985] over the 200 substeps (scenes 0 and 1, FP64
+    oracle), inside the clamp band [1 - 2.5e-2, 1 + 7.5e-3]: the loss is
+    smooth (no plastic-clamp kinks) where finite differences check the
+    gradients.  1.4/s reaches the band's lower edge."""
     scs = [compressing(sc, rate) for sc in scene_batch(seed=seed, nscenes=nscenes, n_range=(n, n), split=n,
                                                         steps=200)]
     rng = np.random.default_rng(seed + 100)

@@ -6,8 +6,10 @@ step 1e-4 for E0 and v0; V0 held fixed as the adjoint holds it).
 
 The scene is a compressing snowball in flight: strains stay inside the
 clamp band and nothing touches a wall, so the loss is smooth where it is
-differentiated.  Tolerance: 2e-3 relative on directional derivatives (the
-adjoint follows the FP32 trajectory, the differences the FP64 one).
+differentiated.  Tolerance: 1e-3 relative on directional derivatives (the
+adjoint follows the FP32 trajectory, the differences the FP64 one; measured
+1.2e-4 for E0 and 2e-6 for v0), and on each spot-checked dL/dx0, whose
+dynamic part (through the transfers and the stress) is also held to 1%.
 """
 import numpy as np
 import pytest
@@ -80,7 +82,7 @@ def test_gradient_wrt_E0(case):
     ad = float(np.sum(case["gE0"].astype(np.float64) * (Ep.astype(np.float64) - Em))) / float(
         np.sum(Ep.astype(np.float64) - Em))
     print(f"E0 directional: adjoint {ad:.6e} fd {fd:.6e}")
-    assert abs(ad - fd) <= 2e-3 * abs(fd), (ad, fd)
+    assert abs(ad - fd) <= 1e-3 * abs(fd), (ad, fd)
 
 
 def test_gradient_wrt_v0(case):
@@ -92,7 +94,7 @@ def test_gradient_wrt_v0(case):
     fd = _fd(case, sc["x"], sc["x"], vp, vm, sc["E0"], sc["E0"])
     ad = float(np.sum(case["gv0"].astype(np.float64) * step))
     print(f"v0 directional: adjoint {ad:.6e} fd {fd:.6e}")
-    assert abs(ad - fd) <= 2e-3 * abs(fd), (ad, fd)
+    assert abs(ad - fd) <= 1e-3 * abs(fd), (ad, fd)
 
 
 def test_gradient_wrt_x0_spot_checks(case):
@@ -109,10 +111,12 @@ def test_gradient_wrt_x0_spot_checks(case):
         fd = _fd(case, xp, xm, sc["v"], sc["v"], sc["E0"], sc["E0"]) / step
         ad = float(case["gx0"][p, k])
         # dL/dx0 = a (the particle carried along) + the dynamic part through
-        # the grid transfers and the stress; hold the dynamic part to 2%
+        # the grid transfers and the stress: the whole within 1e-3, the
+        # dynamic part within 1%
         dyn_fd, dyn_ad = fd - float(case["a"][p, k]), ad - float(case["a"][p, k])
         print(f"x0[{p},{k}]: adjoint {ad:.6e} fd {fd:.6e} dynamic part adjoint {dyn_ad:.4e} fd {dyn_fd:.4e}")
-        assert abs(ad - fd) <= 2e-2 * abs(dyn_fd), (p, k, ad, fd, dyn_fd)
+        assert abs(ad - fd) <= 1e-3 * abs(fd), (p, k, ad, fd)
+        assert abs(ad - fd) <= 1e-2 * abs(dyn_fd), (p, k, ad, fd, dyn_fd)
 
 
 def test_keep_every_substep_matches_checkpoints(case):
