@@ -122,6 +122,44 @@ template <typename T> MPM_HD M3T<T> minverse(const M3T<T>& m) {
   return r;
 }
 
+// ------------------------------------------- fast reciprocal and powers
+// For the stress path only, which feeds the P2G sums (compared to the
+// oracle at 1e-5) and never the bit-exact F update: the hardware's
+// reciprocal (1 ulp) and base-2 log and exponential on the device, the
+// plain expressions in the host build.
+MPM_HD float fast_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+// |x|^(p/3) for p = +-1, x > 0 normal: the polar iteration's scale factor
+MPM_HD float fast_cbrt_pow(float x, float p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_exp2f(p * (1.0f / 3.0f) * __builtin_amdgcn_logf(x));
+#else
+  return exp2f(p * (1.0f / 3.0f) * log2f(x));
+#endif
+}
+// glm's cofactor inverse with the hardware reciprocal of the determinant
+MPM_HD M3 minverse_fast(const M3& m) {
+  const float ood = fast_rcp(+m.c[0][0] * (m.c[1][1] * m.c[2][2] - m.c[2][1] * m.c[1][2]) -
+                             m.c[1][0] * (m.c[0][1] * m.c[2][2] - m.c[2][1] * m.c[0][2]) +
+                             m.c[2][0] * (m.c[0][1] * m.c[1][2] - m.c[1][1] * m.c[0][2]));
+  M3 r;
+  r.c[0][0] = +(m.c[1][1] * m.c[2][2] - m.c[2][1] * m.c[1][2]) * ood;
+  r.c[1][0] = -(m.c[1][0] * m.c[2][2] - m.c[2][0] * m.c[1][2]) * ood;
+  r.c[2][0] = +(m.c[1][0] * m.c[2][1] - m.c[2][0] * m.c[1][1]) * ood;
+  r.c[0][1] = -(m.c[0][1] * m.c[2][2] - m.c[2][1] * m.c[0][2]) * ood;
+  r.c[1][1] = +(m.c[0][0] * m.c[2][2] - m.c[2][0] * m.c[0][2]) * ood;
+  r.c[2][1] = -(m.c[0][0] * m.c[2][1] - m.c[2][0] * m.c[0][1]) * ood;
+  r.c[0][2] = +(m.c[0][1] * m.c[1][2] - m.c[1][1] * m.c[0][2]) * ood;
+  r.c[1][2] = -(m.c[0][0] * m.c[1][2] - m.c[1][0] * m.c[0][2]) * ood;
+  r.c[2][2] = +(m.c[0][0] * m.c[1][1] - m.c[1][0] * m.c[0][1]) * ood;
+  return r;
+}
+
 // ------------------------------------------------- division by a constant
 // a / b correctly rounded, given rb = 1/b correctly rounded (computed once on
 // the host): q = a*rb is within one ulp of a/b, the residual a - b*q is exact
@@ -435,16 +473,19 @@ MPM_HD bool jacobi_svd3(const T (&M)[3][3], T (&U)[3][3], T (&S)[3], T (&V)[3][3
 // Rotation factor R of F = R S by scaled Newton iteration
 // X <- (g X + X^{-T} / g) / 2, g = |det X|^{-1/3}.  Like the reference's
 // Higham–Noferini routine, the result is always a rotation (minus the
-// orthogonal factor when det F < 0).
+// orthogonal factor when det F < 0).  The scale g only speeds convergence
+// (the iteration's fixed point is R for any g > 0), so it and its inverse
+// come from the hardware's log2 / exp2, and X^{-T} from the hardware
+// reciprocal of det X.
 MPM_HD M3 polar_R(const M3& F) {
   M3 X = F;
   const float d0 = mdet(F);
   for (int it = 0; it < 12; ++it) {
     const float d = mdet(X);
-    if (!(fabsf(d) > 0.0f)) break;
-    const M3 Yt = mtrans(minverse(X));
-    const float g = 1.0f / cbrtf(fabsf(d));
-    const float ig = 1.0f / g;
+    if (!(fabsf(d) >= 1.17549435e-38f)) break;  // zero, denormal (the log would be -inf) or NaN
+    const M3 Yt = mtrans(minverse_fast(X));
+    const float g = fast_cbrt_pow(fabsf(d), -1.0f);
+    const float ig = fast_cbrt_pow(fabsf(d), 1.0f);
     float diff = 0.0f;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -555,7 +596,7 @@ MPM_HD M3 stress_matrix(float E, const Lame& L, float V0, const M3& Dinv, const 
   stiff = fmaf(2.0f, mu, lambda);
   const M3 R = polar_R(FE);
   const float J = mdet(FE);
-  const M3 FT = mtrans(minverse(FE));
+  const M3 FT = mtrans(minverse_fast(FE));
   const M3 P = madd(mscale(msub(FE, R), 2.0f * mu), mscale(FT, lambda * (J - 1.0f) * J));
   // (D^-1 V0) P: the diagonal scaling of glm's product (see mul_dinv)
   return mmul(mscale(P, Dinv.c[0][0] * V0), mtrans(FE));
