@@ -143,6 +143,25 @@ int mpm_sync(mpm_ctx* ctx);
 /* Replay runs of substeps through captured HIP graphs (default 1). */
 int mpm_use_graphs(mpm_ctx* ctx, int32_t enable);
 
+/* Reproducible mode (default 0).  The reference sums its grid serially in
+ * particle order (material_point_method.cpp:99-117), so a run repeats
+ * exactly; the device sums with atomics, whose order varies from run to
+ * run.  In this mode every P2G contribution of mpm_substeps and
+ * mpm_init_volumes is rounded to a per-scene quantum first: 2^(e-40) for
+ * mass and 2^(e-38) for momentum, where 2^e is the power of two just above
+ * the scene's largest particle mass (taken again whenever masses are set).
+ * The grid's FP64 sums of such multiples are exact while a node stays below
+ * 2^13 m_ref of mass and 2^15 m_ref m/s of momentum, so they do not depend
+ * on the order of the atomics: results repeat bit for bit from run to run,
+ * for any resort_every and block-pass width, and a scene gives the same
+ * bits alone as inside a batch.  A contribution of 2^-40 m_ref or less is
+ * rounded (to zero below half that), so nodes reached only by stencil
+ * weights below about 1e-12 lose their mass.  The fused pass costs more
+ * (two FP64 adds per contribution).  Not covered: mpm_stage (FP32 force
+ * atomics) and the reverse mode's recording passes. */
+int mpm_set_reproducible(mpm_ctx* ctx, int32_t enable);
+int mpm_get_reproducible(mpm_ctx* ctx, int32_t* enabled);
+
 /* Readback into host arrays (NULL skips a field).  getParticles()
  * (material_point_method.hpp:181-183). */
 int mpm_get_particles(mpm_ctx* ctx, float* x, float* v, float* B, float* FE, float* FP, float* V0);

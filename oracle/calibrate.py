@@ -3,8 +3,8 @@
     python oracle/calibrate.py            # writes oracle/calibration.json
 
 Times the oracle (FP32, one thread, the reference's single-threaded stage
-order) on the workload of the survey's probe of the compiled reference:
-8,192 particles of the configs[0] snowball on a 64^3 grid (h = 1/64) with
+order) on the workload of a measurement of the compiled reference:
+8,192 particles of the reference-sized snowball on a 64^3 grid (h = 1/64) with
 the reference's constants (E = 1.4e5, nu = 0.2, theta_c = 2.5e-2,
 theta_s = 5e-3, xi = 10, dt = 1e-5, no colliders).  The reference measured
 83.3 ms per substep there, 9.8e4 particle-substeps/s on one core of the same
@@ -55,7 +55,7 @@ def main():
     s.substeps(1e-5, steps)
     secs = time.perf_counter() - t0
     rate = n * steps / secs
-    out = dict(workload="configs[0] snowball, 8192 particles, 64^3, reference constants, dt 1e-5, 100 substeps",
+    out = dict(workload="snowball, 8192 particles, 64^3, reference constants, dt 1e-5, 100 substeps",
                oracle_per_core=rate, reference_per_core=REFERENCE_PER_CORE,
                oracle_over_reference=rate / REFERENCE_PER_CORE, ms_per_substep=secs / steps * 1e3,
                cpu=cpu_model(), command="python oracle/calibrate.py")

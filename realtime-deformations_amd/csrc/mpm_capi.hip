@@ -1,5 +1,7 @@
 // C ABI of libmpm_hip: context lifetime, particle/grid transfer, the legacy
 // stage calls and the fused substep loop.  Declarations: include/mpm_c.h.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -227,7 +229,8 @@ KArgs make_args(const mpm_ctx* c, float dt) {
   a.P = c->P;
   a.stride = c->stride;
   a.n = c->n;
-  a.gacc = c->gacc;
+  a.gacc = reinterpret_cast<float4*>(c->gacc);
+  a.gacc64 = c->gacc;
   a.gvel = c->gvel;
   a.gforce = c->gforce;
   a.nnodes = c->nnodes;
@@ -238,6 +241,7 @@ KArgs make_args(const mpm_ctx* c, float dt) {
   a.blk_scene = c->blk_scene;
   a.sc = c->scenes[0];
   a.status = c->status;
+  a.exact = c->exact ? 1 : 0;
   return a;
 }
 void host_collider_inverse(const float t[3], const float q[4], float inv[4][4]) {
@@ -400,10 +404,11 @@ int mpm_create_batch(const mpm_params* params, const int32_t* n_particles, int32
   alloc(&ctx->gcnt, sizeof(int) * 4);
   alloc(&ctx->dstep, sizeof(int));
   alloc(&ctx->bctr, sizeof(int));
-  alloc(&ctx->gacc, gbytes);
+  alloc(&ctx->gacc, 2 * gbytes);  // a double4 per node (reproducible mode)
   alloc(&ctx->gvel, gbytes);
   alloc(&ctx->gforce, gbytes);
   alloc(&ctx->status, sizeof(int) * nscenes);
+  alloc(&ctx->dmref, sizeof(uint32_t) * nscenes);
   alloc(&ctx->d_scenes, sizeof(Scene) * nscenes);
   alloc(&ctx->blk_scene, sizeof(int) * nb);
   if (!ok) {
@@ -420,7 +425,7 @@ int mpm_create_batch(const mpm_params* params, const int32_t* n_particles, int32
   launch_fill_int(ctx->dstep, 1, 1, ctx->stream);
   launch_fill_int(ctx->bctr, 1, 0, ctx->stream);
   launch_fill_int(ctx->nblk, 1, 0, ctx->stream);
-  launch_clear(ctx->gacc, ctx->nnodes, ctx->stream);
+  launch_clear(reinterpret_cast<float4*>(ctx->gacc), 2 * ctx->nnodes, ctx->stream);
   launch_clear(ctx->gvel, ctx->nnodes, ctx->stream);
   launch_clear(ctx->gforce, ctx->nnodes, ctx->stream);
   MPM_HIP(hipMemsetAsync(ctx->status, 0, sizeof(int) * nscenes, ctx->stream));
@@ -444,7 +449,8 @@ int mpm_destroy(mpm_ctx* ctx) {
   void* bufs[] = {ctx->P,       ctx->P2,      ctx->pid,      ctx->pid2,  ctx->keys,     ctx->vals,     ctx->keys2,
                   ctx->vals2,   ctx->hist,    ctx->blk_start, ctx->blk_end, ctx->blk_ids, ctx->blist,   ctx->nblk,
                   ctx->gflag,   ctx->glist[0], ctx->glist[1], ctx->gcnt,  ctx->dstep,    ctx->gacc,     ctx->gvel,
-                  ctx->gforce,  ctx->status,  ctx->dcount,  ctx->d_scenes, ctx->blk_scene, ctx->bctr};
+                  ctx->gforce,  ctx->status,  ctx->dcount,  ctx->d_scenes, ctx->blk_scene, ctx->bctr,
+                  ctx->dmref};
   for (void* b : bufs) (void)hipFree(b);
   for (DevCollider* c : ctx->scoll) (void)hipFree(c);
   for (float* c : ctx->adj_ckpt) (void)hipFree(c);
@@ -504,6 +510,7 @@ int mpm_scene_set_particles(mpm_ctx* ctx, int32_t scene, const float* x, const f
       (rc = upload_field(ctx, scene, FE0, e.data(), 1)))
     return rc;
   ctx->scene_set[scene] = 1;
+  ctx->quanta_stale = true;
   ctx->particles_set = true;
   for (char c : ctx->scene_set) ctx->particles_set = ctx->particles_set && c;
   return MPM_OK;
@@ -588,7 +595,7 @@ int mpm_stage(mpm_ctx* ctx, int32_t stage, float dt) {
   hipStream_t s = ctx->stream;
   switch (stage) {
     case MPM_STAGE_P2G:
-      launch_clear(ctx->gacc, ctx->nnodes, s);
+      launch_clear(reinterpret_cast<float4*>(ctx->gacc), 2 * ctx->nnodes, s);
       launch_clear(ctx->gforce, ctx->nnodes, s);
       launch_stage_p2g(a, s);
       launch_stage_normalise(a, s);
@@ -669,7 +676,7 @@ int rebin(mpm_ctx* ctx) {
 // (after the legacy stages or the initial-volume pass used them).
 void reset_grid(mpm_ctx* ctx) {
   hipStream_t s = ctx->stream;
-  launch_clear(ctx->gacc, ctx->nnodes, s);
+  launch_clear(reinterpret_cast<float4*>(ctx->gacc), 2 * ctx->nnodes, s);
   launch_clear(ctx->gvel, ctx->nnodes, s);
   launch_fill_int(ctx->gflag, ctx->nblocks, 0, s);
   launch_fill_int(ctx->gcnt, 4, 0, s);
@@ -692,6 +699,43 @@ int substep(mpm_ctx* ctx, float dt, bool first) {
   launch_advance_step(ctx->dstep, ctx->gcnt, s);
   ++ctx->since_sort;
   return MPM_OK;
+}
+
+// Drops the captured segment graphs (they bake in the kernel arguments).
+int drop_graphs(mpm_ctx* ctx) {
+  for (hipGraphExec_t& g : ctx->gexec)
+    if (g) {
+      MPM_HIP(hipGraphExecDestroy(g));
+      g = nullptr;
+    }
+  return MPM_OK;
+}
+
+// Reproducible mode: each scene's quanta from its largest particle mass
+// m_ref, after masses changed.  2^e is the power of two just above m_ref,
+// and the mass quantum is 2^(e - 40): node sums stay exact up to 2^13 m_ref
+// of mass and 2^15 m_ref m/s of momentum (quantum 2^(e - 38)), and a
+// particle's largest contribution keeps 40 bits.
+int ensure_quanta(mpm_ctx* ctx) {
+  if (!ctx->exact || !ctx->quanta_stale) return MPM_OK;
+  int max_np = 0;
+  for (const Scene& S : ctx->scenes) max_np = std::max(max_np, S.np);
+  launch_scene_mass_max(ctx->P, ctx->stride, ctx->d_scenes, ctx->nscenes, max_np, ctx->dmref, ctx->stream);
+  MPM_HIP(hipGetLastError());
+  std::vector<uint32_t> bits((size_t)ctx->nscenes);
+  MPM_HIP(hipMemcpyAsync(bits.data(), ctx->dmref, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost, ctx->stream));
+  MPM_HIP(hipStreamSynchronize(ctx->stream));
+  for (int s = 0; s < ctx->nscenes; ++s) {
+    float m;
+    std::memcpy(&m, &bits[s], sizeof m);
+    // no mass (nothing scatters) or a non-finite one (flagged by the pass):
+    // any quantum will do
+    const int e = (m > 0.0f && std::isfinite(m)) ? std::ilogb(m) + 1 : 0;
+    ctx->scenes[s].qexp = e - 40;
+  }
+  if (int rc = upload_scene_table(ctx)) return rc;
+  ctx->quanta_stale = false;
+  return drop_graphs(ctx);
 }
 
 // resort_every fused substeps that start with a re-bin, replayed from a HIP
@@ -756,6 +800,7 @@ int mpm_init_volumes(mpm_ctx* ctx) {
   }
   if (ctx->n == 0) return MPM_OK;
   int rc;
+  if ((rc = ensure_quanta(ctx))) return rc;
   if (!ctx->gacc_clean) reset_grid(ctx);
   if ((rc = rebin(ctx))) return rc;
   const BlockArgs ba = block_args(ctx, 0.0f);
@@ -775,9 +820,10 @@ int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps) {
   }
   if (nsteps == 0 || ctx->n == 0) return MPM_OK;
   hipStream_t s = ctx->stream;
+  int rc;
+  if ((rc = ensure_quanta(ctx))) return rc;
   if (!ctx->gacc_clean) reset_grid(ctx);
   const bool graphs = ctx->use_graphs && !(ctx->profiling && ctx->per_kernel);
-  int rc;
   for (int i = 0; i < nsteps;) {
     if (graphs && i > 0 && ctx->since_sort >= ctx->resort_every && nsteps - i >= ctx->resort_every) {
       if ((rc = graph_segment(ctx, dt))) return rc;
@@ -796,6 +842,20 @@ int mpm_substeps(mpm_ctx* ctx, float dt, int32_t nsteps) {
 int mpm_use_graphs(mpm_ctx* ctx, int32_t enable) {
   if (check_ctx(ctx)) return MPM_ERR_ARG;
   ctx->use_graphs = enable != 0;
+  return MPM_OK;
+}
+
+int mpm_set_reproducible(mpm_ctx* ctx, int32_t enable) {
+  if (check_ctx(ctx)) return MPM_ERR_ARG;
+  if ((enable != 0) == ctx->exact) return MPM_OK;
+  ctx->exact = enable != 0;
+  ctx->quanta_stale = true;
+  return drop_graphs(ctx);
+}
+
+int mpm_get_reproducible(mpm_ctx* ctx, int32_t* enabled) {
+  if (check_ctx(ctx) || !enabled) return MPM_ERR_ARG;
+  *enabled = ctx->exact ? 1 : 0;
   return MPM_OK;
 }
 

@@ -170,14 +170,17 @@ __device__ __forceinline__ void rot4(float (&w)[4], int r) {
 }
 
 // G2P of one particle from the LDS halo, or from global memory when its
-// stencil left the halo.
+// stencil left the halo.  PLAIN: sum the 64 nodes in the plain order
+// whatever the lane (reproducible mode: the rotated order depends on the
+// slot a binning gave the particle, and FP32 sums on their order).
+template <int PLAIN>
 __device__ __forceinline__ void g2p_one(const KArgs& a, const Scene& S, const float* __restrict__ vh, const int o[3],
                                         F3 x, F3& v, M3& B) {
   StencilInfo s = stencil_at(S, o, x);
   float dd[3][4];
   if (s.fits) {
-    int r[3];
-    lane_rotation(s, r);
+    int r[3] = {0, 0, 0};
+    if (!PLAIN) lane_rotation(s, r);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const float xc = d == 0 ? x.x : (d == 1 ? x.y : x.z);
@@ -245,14 +248,16 @@ __device__ __forceinline__ void p2g_scatter(const Scene& S, const StencilInfo& s
   }
 }
 
+template <int EXACT>
 __device__ __forceinline__ void scatter_global(const BlockArgs& ba, const Scene& S, const StencilInfo& s, F3 x, float m,
                                             F3 mv, const M3& Q);
 
 // Scatter into the LDS halo, or with global atomics (and the touched grid
-// blocks marked) when the stencil left the halo.
+// blocks marked) when the stencil left the halo.  EXACT: each contribution
+// rounded to the scene's quanta first (reproducible mode).
+template <int EXACT>
 __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S, double* __restrict__ ah,
                                             const int o[3], F3 x, float m, F3 mv, const M3& Q) {
-  const KArgs& a = ba.a;
   StencilInfo s = stencil_at(S, o, x);
   if (s.fits) {
     int r[3];
@@ -268,20 +273,33 @@ __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S,
       qy[k] = (s.lb[1] + ((k + r[1]) & 3)) * SY;
       qz[k] = s.lb[2] + ((k + r[2]) & 3);
     }
-    p2g_scatter(S, s, r, x, m, mv, Q, [&](int ia, int ib, int ic, int, int, int, float vm, float px, float py, float pz) {
-      const int q = qx[ia] + qy[ib] + qz[ic];
-      atomicAdd(&ah[q], (double)vm);
-      atomicAdd(&ah[PLANE + q], (double)px);
-      atomicAdd(&ah[2 * PLANE + q], (double)py);
-      atomicAdd(&ah[3 * PLANE + q], (double)pz);
-    });
+    if (EXACT) {
+      const double Mm = quantum_magic(S.qexp), Mp = quantum_magic(S.qexp + 2);
+      p2g_scatter(S, s, r, x, m, mv, Q, [&](int ia, int ib, int ic, int, int, int, float vm, float px, float py, float pz) {
+        const int q = qx[ia] + qy[ib] + qz[ic];
+        atomicAdd(&ah[q], on_quantum((double)vm, Mm));
+        atomicAdd(&ah[PLANE + q], on_quantum((double)px, Mp));
+        atomicAdd(&ah[2 * PLANE + q], on_quantum((double)py, Mp));
+        atomicAdd(&ah[3 * PLANE + q], on_quantum((double)pz, Mp));
+      });
+    } else {
+      p2g_scatter(S, s, r, x, m, mv, Q, [&](int ia, int ib, int ic, int, int, int, float vm, float px, float py, float pz) {
+        const int q = qx[ia] + qy[ib] + qz[ic];
+        atomicAdd(&ah[q], (double)vm);
+        atomicAdd(&ah[PLANE + q], (double)px);
+        atomicAdd(&ah[2 * PLANE + q], (double)py);
+        atomicAdd(&ah[3 * PLANE + q], (double)pz);
+      });
+    }
   } else {
-    scatter_global(ba, S, s, x, m, mv, Q);
+    scatter_global<EXACT>(ba, S, s, x, m, mv, Q);
   }
 }
 
 // The global-memory scatter of a particle whose stencil left its block's
-// halo: FP32 atomics into the grid accumulators, touched grid blocks marked.
+// halo: atomics into the grid accumulators (FP32, or FP64 on the quanta in
+// reproducible mode), touched grid blocks marked.
+template <int EXACT>
 __device__ __forceinline__ void scatter_global(const BlockArgs& ba, const Scene& S, const StencilInfo& s, F3 x, float m,
                                             F3 mv, const M3& Q) {
   const KArgs& a = ba.a;
@@ -294,12 +312,21 @@ __device__ __forceinline__ void scatter_global(const BlockArgs& ba, const Scene&
                      (s.ax[2].base - 1 + 3 * dk) >> 2);
     const int r0[3] = {0, 0, 0};
     p2g_scatter(S, s, r0, x, m, mv, Q, [&](int, int, int, int ia, int ib, int ic, float vm, float px, float py, float pz) {
-      float* g = reinterpret_cast<float*>(
-          a.gacc + node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, S));
-      atomicAdd(g + 0, vm);
-      atomicAdd(g + 1, px);
-      atomicAdd(g + 2, py);
-      atomicAdd(g + 3, pz);
+      const int64_t gi = node_index(s.ax[0].base - 1 + ia, s.ax[1].base - 1 + ib, s.ax[2].base - 1 + ic, S);
+      if (EXACT) {
+        double* g = reinterpret_cast<double*>(a.gacc64 + gi);
+        const double Mm = quantum_magic(S.qexp), Mp = quantum_magic(S.qexp + 2);
+        atomicAdd(g + 0, on_quantum((double)vm, Mm));
+        atomicAdd(g + 1, on_quantum((double)px, Mp));
+        atomicAdd(g + 2, on_quantum((double)py, Mp));
+        atomicAdd(g + 3, on_quantum((double)pz, Mp));
+      } else {
+        float* g = reinterpret_cast<float*>(a.gacc + gi);
+        atomicAdd(g + 0, vm);
+        atomicAdd(g + 1, px);
+        atomicAdd(g + 2, py);
+        atomicAdd(g + 3, pz);
+      }
     });
   }
 }
@@ -365,13 +392,13 @@ __device__ __forceinline__ void particle_update(const BlockArgs& ba, const Scene
 }
 
 // Stress, F update, then the per-particle P2G scatter.
-template <int COMPOSE, int REC>
+template <int COMPOSE, int REC, int EXACT>
 __device__ __forceinline__ void p2g_one(const BlockArgs& ba, const Scene& S, int scene, double* __restrict__ ah,
                                         const int o[3], int p, F3 x, F3 v, const M3& B) {
   float m;
   M3 Q;
   particle_update<COMPOSE, REC>(ba, S, scene, p, x, v, B, m, Q);
-  scatter_one(ba, S, ah, o, x, m, v * m, Q);
+  scatter_one<EXACT>(ba, S, ah, o, x, m, v * m, Q);
 }
 
 __device__ __forceinline__ F3 advect(const KArgs& a, const Scene& S, F3 x, F3 v) {
@@ -388,7 +415,9 @@ __device__ __forceinline__ F3 advect(const KArgs& a, const Scene& S, F3 x, F3 v)
 // device table, copied once per block.
 // LITERAL kernels are held to three waves per SIMD (<= 168 registers); the
 // TEXTBOOK ones, with their FP64 re-solve, would spill heavily and keep two.
-template <int MODE, int COMPOSE, int BATCHED, int REC = 0>
+// EXACT: reproducible mode, the P2G contributions on the scene's quanta and
+// G2P summed in the plain node order.
+template <int MODE, int COMPOSE, int BATCHED, int REC = 0, int EXACT = 0>
 __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 1) k_block_pass(BlockArgs ba) {
   constexpr bool kG2P = MODE == MODE_FUSED || MODE == MODE_G2P;
   constexpr bool kP2G = MODE != MODE_G2P;
@@ -430,13 +459,13 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 1) k_blo
       F3 x = f3(ld(a, FX, p), ld(a, FX + 1, p), ld(a, FX + 2, p));
       if (MODE == MODE_MASS) {
         M3 Z = mdiag(0.0f);
-        scatter_one(ba, S, ah, o, x, ld(a, FM, p), f3(0.f, 0.f, 0.f), Z);
+        scatter_one<EXACT>(ba, S, ah, o, x, ld(a, FM, p), f3(0.f, 0.f, 0.f), Z);
         continue;
       }
       F3 v;
       M3 B;
       if (kG2P) {
-        g2p_one(a, S, vh, o, x, v, B);
+        g2p_one<EXACT>(a, S, vh, o, x, v, B);
         x = advect(a, S, x, v);
         st(a, FX, p, x.x);
         st(a, FX + 1, p, x.y);
@@ -462,22 +491,37 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 1) k_blo
 #pragma unroll
           for (int r = 0; r < 3; ++r) B.c[c][r] = ld(a, FB + 3 * c + r, p);
       }
-      if (kP2G) p2g_one<COMPOSE, REC>(ba, S, scene, ah, o, p, x, v, B);
+      if (kP2G) p2g_one<COMPOSE, REC, EXACT>(ba, S, scene, ah, o, p, x, v, B);
     }
     __syncthreads();
     if (kP2G) {
       for (int q = t; q < HN * HN * HN; q += TPB) {
         const int hx = q / (HN * HN), hy = (q / HN) % HN, hz = q % HN;
         const int l = hx * SX + hy * SY + hz;
-        const float m = (float)ah[l], px = (float)ah[PLANE + l], py = (float)ah[2 * PLANE + l], pz = (float)ah[3 * PLANE + l];
-        if (m != 0.0f || px != 0.0f || py != 0.0f || pz != 0.0f) {
-          const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
-          if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2]) {
-            float* g = reinterpret_cast<float*>(a.gacc + node_index(gi, gj, gk, S));
-            atomicAdd(g + 0, m);
-            atomicAdd(g + 1, px);
-            atomicAdd(g + 2, py);
-            atomicAdd(g + 3, pz);
+        if (EXACT) {  // exact FP64 sums of multiples of the quanta
+          const double m = ah[l], px = ah[PLANE + l], py = ah[2 * PLANE + l], pz = ah[3 * PLANE + l];
+          if (m != 0.0 || px != 0.0 || py != 0.0 || pz != 0.0) {
+            const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
+            if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2]) {
+              double* g = reinterpret_cast<double*>(a.gacc64 + node_index(gi, gj, gk, S));
+              atomicAdd(g + 0, m);
+              atomicAdd(g + 1, px);
+              atomicAdd(g + 2, py);
+              atomicAdd(g + 3, pz);
+            }
+          }
+        } else {
+          const float m = (float)ah[l], px = (float)ah[PLANE + l], py = (float)ah[2 * PLANE + l],
+                      pz = (float)ah[3 * PLANE + l];
+          if (m != 0.0f || px != 0.0f || py != 0.0f || pz != 0.0f) {
+            const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
+            if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2]) {
+              float* g = reinterpret_cast<float*>(a.gacc + node_index(gi, gj, gk, S));
+              atomicAdd(g + 0, m);
+              atomicAdd(g + 1, px);
+              atomicAdd(g + 2, py);
+              atomicAdd(g + 3, pz);
+            }
           }
         }
       }
@@ -670,7 +714,21 @@ template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(Bloc
     if (s < ncur) {
       const int gb = glist_cur[s];
       const int64_t g = (int64_t)gb * 64 + lane;
-      const float4 acc = a.gacc[g];
+      // the sums as FP32, as the grid update uses them; the accumulators
+      // are cleared below in their own format (an FP64 sum can round to
+      // an FP32 zero)
+      bool touched;
+      float4 acc;
+      if (a.exact) {
+        const double4 d = a.gacc64[g];
+        acc = make_float4((float)d.x, (float)d.y, (float)d.z, (float)d.w);
+        touched = d.x != 0.0 || d.y != 0.0 || d.z != 0.0 || d.w != 0.0;
+        if (touched) a.gacc64[g] = make_double4(0.0, 0.0, 0.0, 0.0);
+      } else {
+        acc = a.gacc[g];
+        touched = acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f || acc.w != 0.0f;
+        if (touched) a.gacc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
       if (ba.grec.acc) {
         ba.grec.acc[(int64_t)s * 64 + lane] = acc;
         if (lane == 0) ba.grec.list[1 + s] = gb;
@@ -692,8 +750,6 @@ template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(Bloc
         }
         out = make_float4(v.x, v.y, v.z, acc.x);
       }
-      if (acc.x != 0.0f || acc.y != 0.0f || acc.z != 0.0f || acc.w != 0.0f)
-        a.gacc[g] = make_float4(0.f, 0.f, 0.f, 0.f);
       a.gvel[g] = out;
     } else {
       const int gb = glist_prev[s - ncur];
@@ -724,6 +780,24 @@ template <int BATCHED> void launch_block_pass_t(const BlockArgs& ba, int mode, h
       default: k_block_pass<MODE_G2P, COMPOSE_TEXTBOOK, BATCHED, 1><<<grid, TPB, 0, s>>>(ba); break;
     }
     return;
+  }
+  if (ba.a.exact) {  // reproducible mode: P2G on the quanta, G2P in the plain order
+    switch (mode) {
+      case MODE_P2G:
+        if (lit)
+          k_block_pass<MODE_P2G, COMPOSE_LITERAL, BATCHED, 0, 1><<<grid, TPB, 0, s>>>(ba);
+        else
+          k_block_pass<MODE_P2G, COMPOSE_TEXTBOOK, BATCHED, 0, 1><<<grid, TPB, 0, s>>>(ba);
+        return;
+      case MODE_FUSED:
+        if (lit)
+          k_block_pass<MODE_FUSED, COMPOSE_LITERAL, BATCHED, 0, 1><<<grid, TPB, 0, s>>>(ba);
+        else
+          k_block_pass<MODE_FUSED, COMPOSE_TEXTBOOK, BATCHED, 0, 1><<<grid, TPB, 0, s>>>(ba);
+        return;
+      case MODE_MASS: k_block_pass<MODE_MASS, COMPOSE_LITERAL, BATCHED, 0, 1><<<grid, TPB, 0, s>>>(ba); return;
+      default: k_block_pass<MODE_G2P, COMPOSE_LITERAL, BATCHED, 0, 1><<<grid, TPB, 0, s>>>(ba); return;
+    }
   }
   switch (mode) {
     case MODE_P2G:

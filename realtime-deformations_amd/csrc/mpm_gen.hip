@@ -461,6 +461,7 @@ int mpm_scene_generate(mpm_ctx* ctx, int32_t scene, const mpm_generator* g, int6
     GEN_FAIL(MPM_ERR_BOUNDS, "a generated particle puts its stencil outside the scene's grid");
   }
   ctx->scene_set[scene] = 1;
+  ctx->quanta_stale = true;
   ctx->particles_set = true;
   for (char c : ctx->scene_set) ctx->particles_set = ctx->particles_set && c;
 done:

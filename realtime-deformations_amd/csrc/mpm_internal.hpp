@@ -80,6 +80,10 @@ struct Scene {
   float g[3];
   int bc_mode, ncoll;
   const DevCollider* coll;
+  // reproducible mode: P2G contributions are rounded to multiples of
+  // 2^qexp (mass) and 2^(qexp + 2) (momentum), 2^qexp = 2^-40 of the
+  // scene's largest particle mass rounded up to a power of two
+  int qexp;
 };
 
 // Everything a kernel needs, passed by value.
@@ -87,7 +91,8 @@ struct KArgs {
   float* P;
   int64_t stride;
   int n;           // particles over all scenes
-  float4* gacc;    // (m, p) accumulators
+  float4* gacc;    // (m, p) accumulators, FP32 (default mode)
+  double4* gacc64; // the same memory as FP64 accumulators (reproducible mode)
   float4* gvel;    // (v, m) after the grid update
   float4* gforce;  // (f, 0), legacy stage path only
   int64_t nnodes;  // nodes over all scenes
@@ -98,7 +103,27 @@ struct KArgs {
   const int* blk_scene;  // grid block -> scene
   Scene sc;              // scene 0 by value (legacy stage kernels: one scene)
   int* status;           // per scene
+  int exact;             // reproducible mode (P2G sums on the scenes' quanta)
 };
+
+// Reproducible P2G sums.  M = quantum_magic(k) = 1.5 * 2^(52 + k); for
+// |c| < 2^(51 + k), c + M lies in [2^(52 + k), 2^(53 + k)), where doubles are
+// spaced by 2^k, so on_quantum(c, M) is c rounded to the nearest multiple of
+// 2^k (subtracting M is exact).  A sum of such multiples is exact, and so
+// the same in any order, while it stays below 2^(53 + k) in magnitude;
+// beyond that it is rounded like any FP64 sum.  Non-finite c stays
+// non-finite.  (Built without fast-math: the add and subtract are kept.)
+__device__ __forceinline__ double quantum_magic(int k) { return ldexp(1.5, 52 + k); }
+__device__ __forceinline__ double on_quantum(double c, double M) { return (c + M) - M; }
+
+// P2G sums of node g as FP32, from the accumulators of the mode
+__device__ __forceinline__ float4 load_acc(const KArgs& a, int64_t g) {
+  if (a.exact) {
+    const double4 d = a.gacc64[g];
+    return make_float4((float)d.x, (float)d.y, (float)d.z, (float)d.w);
+  }
+  return a.gacc[g];
+}
 
 __device__ __forceinline__ int64_t node_index(int i, int j, int k, const Scene& S) {
   return (S.blk0 + ((int64_t)((i >> 2) * S.nb[1] + (j >> 2)) * S.nb[2] + (k >> 2))) * 64 + ((i & 3) << 4) +
@@ -148,7 +173,8 @@ struct mpm_ctx {
   int since_sort = -1;      // substeps since the last binning (-1: never binned)
   int resort_every = 16;
   int block_grid = 2048;    // workgroups of the persistent block pass
-  float4* gacc = nullptr;
+  double4* gacc = nullptr;  // (m, p) accumulators: FP64 in reproducible mode,
+                            // else FP32 in the first half
   float4* gvel = nullptr;
   float4* gforce = nullptr;
   int* status = nullptr;    // device status word per scene
@@ -156,6 +182,9 @@ struct mpm_ctx {
   std::vector<char> scene_set;  // particles uploaded, per scene
   bool particles_set = false;   // every scene has particles
   bool gacc_clean = false;      // fused path needs zeroed accumulators
+  bool exact = false;           // reproducible mode
+  bool quanta_stale = true;     // masses changed since the scenes' quanta were set
+  uint32_t* dmref = nullptr;    // per scene: bits of its largest particle mass
   std::string err;
   // HIP graphs of [rebin + resort_every substeps], one per particle-buffer
   // parity, valid for one dt (scene data, colliders included, is read from
@@ -240,6 +269,8 @@ void host_collider_inverse(const float t[3], const float q[4], float inv[4][4]);
 void launch_count_active(const KArgs& a, int64_t* out, hipStream_t s);
 // batched initial volumes: density gathered from the node masses in gvel.w
 void launch_volumes(const KArgs& a, hipStream_t s);
+void launch_scene_mass_max(const float* P, int64_t stride, const Scene* scenes, int nscenes, int max_np, uint32_t* out,
+                           hipStream_t s);
 // one field of one scene between the caller-ordered host array and the
 // particle rows (synchronous); NULL host skips
 int upload_field(mpm_ctx* ctx, int scene, int field, const float* host, int comps);

@@ -427,6 +427,7 @@ int mpm_snapshot_load(mpm_ctx* ctx, const char* path, int64_t* step) {
       if ((rc = upload_field(ctx, s, BLOCKS[b].field, data[s * 8 + b].data(), BLOCKS[b].comps))) return rc;
     if ((rc = mpm_scene_set_colliders(ctx, s, m.coll[s].data(), (int32_t)m.coll[s].size()))) return rc;
     ctx->scene_set[s] = 1;
+    ctx->quanta_stale = true;
   }
   ctx->particles_set = true;
   if (step) *step = m.h.step;

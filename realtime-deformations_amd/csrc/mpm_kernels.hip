@@ -71,11 +71,21 @@ __global__ void k_stage_p2g(KArgs a) {
         const float w = s.ax[0].w[dx] * s.ax[1].w[dy] * s.ax[2].w[dz];
         const F3 Xi = f3((float)i * a.sc.h, (float)j * a.sc.h, (float)k * a.sc.h);
         const F3 mom = (v + mvec(BD, Xi - x)) * (w * m);
-        float* g = reinterpret_cast<float*>(a.gacc + node_index(i, j, k, a.sc));
-        atomicAdd(g + 0, m * w);
-        atomicAdd(g + 1, mom.x);
-        atomicAdd(g + 2, mom.y);
-        atomicAdd(g + 3, mom.z);
+        const int64_t gi = node_index(i, j, k, a.sc);
+        if (a.exact) {  // reproducible mode: on the scene's quanta, FP64
+          double* g = reinterpret_cast<double*>(a.gacc64 + gi);
+          const double Mm = quantum_magic(a.sc.qexp), Mp = quantum_magic(a.sc.qexp + 2);
+          atomicAdd(g + 0, on_quantum((double)(m * w), Mm));
+          atomicAdd(g + 1, on_quantum((double)mom.x, Mp));
+          atomicAdd(g + 2, on_quantum((double)mom.y, Mp));
+          atomicAdd(g + 3, on_quantum((double)mom.z, Mp));
+        } else {
+          float* g = reinterpret_cast<float*>(a.gacc + gi);
+          atomicAdd(g + 0, m * w);
+          atomicAdd(g + 1, mom.x);
+          atomicAdd(g + 2, mom.y);
+          atomicAdd(g + 3, mom.z);
+        }
       }
 }
 
@@ -97,7 +107,7 @@ __device__ __forceinline__ bool node_coords(const Scene& S, int64_t g, int& i, i
 __global__ void k_stage_normalise(KArgs a) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.nnodes) return;
-  const float4 acc = a.gacc[g];
+  const float4 acc = load_acc(a, g);
   float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
   if (acc.x != 0.0f) out = make_float4(acc.y / acc.x, acc.z / acc.x, acc.w / acc.x, acc.x);
   a.gvel[g] = out;
@@ -369,6 +379,24 @@ void launch_grid_to_dense(const KArgs& a, const Scene& S, float* m, float* vel, 
 }
 void launch_volumes(const KArgs& a, hipStream_t s) {
   if (a.n) k_volumes<<<nblk(a.n), TPB, 0, s>>>(a);
+}
+
+// out[scene] = bits of the largest |m| over the scene's particle slots
+// (blockIdx.y: scene; out cleared by the caller).  The bits of non-negative
+// floats order like the floats; a NaN mass orders above every finite one.
+__global__ void k_scene_mass_max(const float* P, int64_t stride, const Scene* scenes, uint32_t* out) {
+  const Scene& S = scenes[blockIdx.y];
+  uint32_t best = 0;
+  for (int p = blockIdx.x * TPB + threadIdx.x; p < S.np; p += gridDim.x * TPB)
+    best = max(best, __float_as_uint(fabsf(P[(int64_t)FM * stride + S.p0 + p])));
+  for (int off = 32; off > 0; off >>= 1) best = max(best, (uint32_t)__shfl_xor((int)best, off));
+  if ((threadIdx.x & 63) == 0 && best) atomicMax(out + blockIdx.y, best);
+}
+void launch_scene_mass_max(const float* P, int64_t stride, const Scene* scenes, int nscenes, int max_np, uint32_t* out,
+                           hipStream_t s) {
+  launch_fill_int(reinterpret_cast<int*>(out), nscenes, 0, s);
+  const int gx = std::max(1, std::min(nblk(max_np), 1024));
+  k_scene_mass_max<<<dim3(gx, nscenes), TPB, 0, s>>>(P, stride, scenes, out);
 }
 
 }  // namespace mpm

@@ -89,6 +89,7 @@ EXPORTS = [
     "mpm_set_particles", "mpm_set_state", "mpm_set_colliders", "mpm_init_volumes", "mpm_stage",
     "mpm_substeps", "mpm_sync", "mpm_get_particles", "mpm_get_grid", "mpm_profile_begin", "mpm_profile_end",
     "mpm_count_active", "mpm_debug_bin", "mpm_debug_blocks", "mpm_copy_probe", "mpm_use_graphs",
+    "mpm_set_reproducible", "mpm_get_reproducible",
     "mpm_create_batch", "mpm_num_scenes", "mpm_scene_set_particles", "mpm_scene_set_state",
     "mpm_scene_set_colliders", "mpm_scene_get_particles", "mpm_scene_get_grid", "mpm_scene_status",
     "mpm_set_tuning", "mpm_get_tuning", "mpm_autotune", "mpm_adjoint_forward", "mpm_adjoint_interval", "mpm_adjoint_backward",
@@ -125,6 +126,8 @@ def lib():
         L.mpm_get_grid.argtypes = [vp, _F, _F, _F, C.POINTER(C.c_uint8)]
         L.mpm_profile_begin.argtypes = [vp, C.c_int32]
         L.mpm_use_graphs.argtypes = [vp, C.c_int32]
+        L.mpm_set_reproducible.argtypes = [vp, C.c_int32]
+        L.mpm_get_reproducible.argtypes = [vp, C.POINTER(C.c_int32)]
         L.mpm_create_batch.argtypes = [C.POINTER(Params), C.POINTER(C.c_int32), C.c_int32, C.POINTER(vp)]
         L.mpm_num_scenes.argtypes = [vp]
         L.mpm_scene_set_particles.argtypes = [vp, C.c_int32, _F, _F, _F, _F]
@@ -278,6 +281,18 @@ class Simulation:
     def use_graphs(self, enable=True):
         """Replay runs of substeps from captured HIP graphs (the default)."""
         self._check(self.L.mpm_use_graphs(self.ctx, int(bool(enable))))
+
+    def set_reproducible(self, enable=True):
+        """Reproducible mode: P2G contributions rounded to per-scene quanta,
+        so the grid sums, and every result, repeat bit for bit whatever the
+        order of the atomics, the binning interval or the batch."""
+        self._check(self.L.mpm_set_reproducible(self.ctx, int(bool(enable))))
+
+    @property
+    def reproducible(self):
+        out = C.c_int32()
+        self._check(self.L.mpm_get_reproducible(self.ctx, C.byref(out)))
+        return bool(out.value)
 
     def profile_begin(self, per_kernel=True):
         """Start a profiled region.  per_kernel brackets every launch with HIP
@@ -446,6 +461,20 @@ class Simulation:
         gx0, gv0, gE0 = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
         self._check(self.L.mpm_adjoint_backward(self.ctx, _fp(gx), _fp(gv), _fp(gx0), _fp(gv0), _fp(gE0)))
         return gx0, gv0, gE0
+
+    def scene_grid(self, scene, force=False):
+        """Dense grid of one scene: m, vel (3), active (and force, stage
+        path only)."""
+        cells = int(np.prod(tuple(self.scene_params[scene].dims)))
+        m, vel = np.zeros(cells, np.float32), np.zeros((cells, 3), np.float32)
+        f = np.zeros((cells, 3), np.float32) if force else None
+        act = np.zeros(cells, np.uint8)
+        self._check(self.L.mpm_scene_get_grid(self.ctx, int(scene), _fp(m), _fp(vel), _fp(f),
+                                              act.ctypes.data_as(C.POINTER(C.c_uint8))))
+        out = dict(m=m, vel=vel, active=act.astype(bool))
+        if force:
+            out["force"] = f
+        return out
 
     def grid(self, force=False):
         cells = int(np.prod(self.dims))

@@ -104,14 +104,14 @@ def _check(errs, flips, n, compose, tol=1e-3):
 
 @pytest.mark.parametrize("compose", [M.COMPOSE_TEXTBOOK, M.COMPOSE_LITERAL], ids=["textbook", "literal"])
 def test_two_snowballs_through_contact(compose):
-    """262,144 particles in two snowballs at +-3 m/s on 128^3,
+    """The two-snowball collision: 262,144 particles in two snowballs at +-3 m/s on 128^3,
     dt 5e-5.  The GPU runs to substep 620 (the spheres touch near substep
     667), then GPU and oracle run 100 substeps of the collision from that
     state."""
     sc = scenes.two_snowballs()
     st = _gpu_state(sc, compose, 620)
     errs, flips, deformed, plastic, status = _from_state(sc, st, compose, 100)
-    print(f"configs[1] 620->720 {'textbook' if compose else 'literal'}: deformed {deformed:.3f} plastic "
+    print(f"two snowballs 620->720 {'textbook' if compose else 'literal'}: deformed {deformed:.3f} plastic "
           f"{plastic:.3f} flips {flips} status {status} errs " + " ".join(f"{k} {v:.2e}" for k, v in errs.items()))
     assert deformed > 0.05, deformed  # the comparison window is in contact
     assert status[0] in (0, M.MPM_ERR_CFL)  # no error; a CFL warning is reported above
@@ -120,7 +120,7 @@ def test_two_snowballs_through_contact(compose):
 
 @pytest.mark.parametrize("compose", [M.COMPOSE_TEXTBOOK, M.COMPOSE_LITERAL], ids=["textbook", "literal"])
 def test_snowball_landing_on_the_sticky_floor(compose):
-    """The configs[0] snowball (8,192 particles, 64^3, dt 1e-4) started three
+    """The 8,192-particle snowball (64^3, dt 1e-4) started three
     cells above the sticky floor band at 2 m/s: its bottom reaches the band's
     stencils (y < 4h) near substep 151, so the GPU runs 140 substeps, then GPU
     and oracle run the 100 substeps of the landing from that state (the floor
