@@ -95,7 +95,7 @@ def cpu_baseline(sc, V0, threads):
     reference runs, then four on `threads` OpenMP threads; about 25 s of host
     time in all.  oracle/calibration.json converts the rate per core into the
     reference's own speed (the oracle against the compiled reference's 9.8e4
-    particle-substeps/s per core on the survey's probe workload)."""
+    particle-substeps/s per core, measured on the same workload)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 

@@ -170,30 +170,30 @@ __device__ __forceinline__ void rot4(float (&w)[4], int r) {
 }
 
 // G2P of one particle from the LDS halo, or from global memory when its
-// stencil left the halo.  PLAIN: sum the 64 nodes in the plain order
-// whatever the lane (reproducible mode: the rotated order depends on the
-// slot a binning gave the particle, and FP32 sums on their order).
-template <int PLAIN>
+// stencil left the halo, summing the 64 nodes in the plain (i, j, k) order
+// of the reference whatever the lane.  Reads, unlike the scatter's atomics,
+// gain nothing from the per-lane rotation: lanes of one cell read one
+// address (a broadcast), and in the plain order the compiler merges the
+// reads of neighbouring z nodes (20% fewer LDS instructions, 2% faster
+// passes than rotated).  The order also no longer depends on the slot a
+// binning gave the particle, which the reproducible mode needs.
 __device__ __forceinline__ void g2p_one(const KArgs& a, const Scene& S, const float* __restrict__ vh, const int o[3],
                                         F3 x, F3& v, M3& B) {
   StencilInfo s = stencil_at(S, o, x);
   float dd[3][4];
   if (s.fits) {
-    int r[3] = {0, 0, 0};
-    if (!PLAIN) lane_rotation(s, r);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const float xc = d == 0 ? x.x : (d == 1 ? x.y : x.z);
-      rot4(s.ax[d].w, r[d]);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) dd[d][k] = (float)(s.ax[d].base - 1 + ((k + r[d]) & 3)) * S.h - xc;
+      for (int k = 0; k < 4; ++k) dd[d][k] = (float)(s.ax[d].base - 1 + k) * S.h - xc;
     }
-    int qx[4], qy[4], qz[4];  // halo offsets per traversal step and axis
+    int qx[4], qy[4], qz[4];  // halo offsets per node and axis
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      qx[k] = (s.lb[0] + ((k + r[0]) & 3)) * SX;
-      qy[k] = (s.lb[1] + ((k + r[1]) & 3)) * SY;
-      qz[k] = s.lb[2] + ((k + r[2]) & 3);
+      qx[k] = (s.lb[0] + k) * SX;
+      qy[k] = (s.lb[1] + k) * SY;
+      qz[k] = s.lb[2] + k;
     }
     g2p_accum(s.ax, dd, [&](int ia, int ib, int ic) {
       const int q = qx[ia] + qy[ib] + qz[ic];
@@ -417,8 +417,7 @@ __device__ __forceinline__ F3 advect(const KArgs& a, const Scene& S, F3 x, F3 v)
 // TEXTBOOK ones, with their FP64 re-solve, would spill heavily there and are
 // held to two (<= 256 registers: the batched fused pass took 258 unbounded,
 // one wave per SIMD, and ran 1.5x slower).
-// EXACT: reproducible mode, the P2G contributions on the scene's quanta and
-// G2P summed in the plain node order.
+// EXACT: reproducible mode, the P2G contributions on the scene's quanta.
 template <int MODE, int COMPOSE, int BATCHED, int REC = 0, int EXACT = 0>
 __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 2) k_block_pass(BlockArgs ba) {
   constexpr bool kG2P = MODE == MODE_FUSED || MODE == MODE_G2P;
@@ -467,7 +466,7 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 2) k_blo
       F3 v;
       M3 B;
       if (kG2P) {
-        g2p_one<EXACT>(a, S, vh, o, x, v, B);
+        g2p_one(a, S, vh, o, x, v, B);
         x = advect(a, S, x, v);
         st(a, FX, p, x.x);
         st(a, FX + 1, p, x.y);
@@ -783,7 +782,7 @@ template <int BATCHED> void launch_block_pass_t(const BlockArgs& ba, int mode, h
     }
     return;
   }
-  if (ba.a.exact) {  // reproducible mode: P2G on the quanta, G2P in the plain order
+  if (ba.a.exact) {  // reproducible mode: P2G on the quanta
     switch (mode) {
       case MODE_P2G:
         if (lit)
@@ -798,7 +797,7 @@ template <int BATCHED> void launch_block_pass_t(const BlockArgs& ba, int mode, h
           k_block_pass<MODE_FUSED, COMPOSE_TEXTBOOK, BATCHED, 0, 1><<<grid, TPB, 0, s>>>(ba);
         return;
       case MODE_MASS: k_block_pass<MODE_MASS, COMPOSE_LITERAL, BATCHED, 0, 1><<<grid, TPB, 0, s>>>(ba); return;
-      default: k_block_pass<MODE_G2P, COMPOSE_LITERAL, BATCHED, 0, 1><<<grid, TPB, 0, s>>>(ba); return;
+      default: break;  // G2P scatters nothing
     }
   }
   switch (mode) {
