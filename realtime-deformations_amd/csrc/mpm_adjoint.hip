@@ -212,16 +212,36 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
       for (int r = 0; r < 3; ++r) Q.c[c][r] = m * BD.c[c][r] - a.dt * A.c[c][r];
     mv = v * m;
   }
-  // gather the node adjoints
+  // gather the node adjoints.  Per node i (weights w_i, derivatives dw_i,
+  // d_i = x_i - x_p) the adjoint of P2G contributes
+  //   gx += dw_i s_i - w_i Q^T gP_i,  s_i = m gm_i + gP_i . (m v + Q d_i)
+  //   gv += m w_i gP_i,               gQ += w_i gP_i d_i^T
+  // The weights are products over the axes, so every sum but s_i's is
+  // separable: inner sums over z, then y, then x, as the forward G2P does
+  // (about 20 vector instructions per node instead of 55), and
+  // sum w_i Q^T gP_i = Q^T sum w_i gP_i.
   AxisD ax[3];
   ax[0] = axis_d(x.x, S, S.dims[0] - 3);
   ax[1] = axis_d(x.y, S, S.dims[1] - 3);
   ax[2] = axis_d(x.z, S, S.dims[2] - 3);
-  F3 gv = f3(0.f, 0.f, 0.f);
-  F3 gx = f3(fld(Gout, sd, AX, p), fld(Gout, sd, AX + 1, p), fld(Gout, sd, AX + 2, p));
-  M3 gQ = mzero();
+  float dd[3][4];  // d_i per axis, rounded as the forward rounds them
+  {
+    const float xc[3] = {x.x, x.y, x.z};
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) dd[e][k] = (float)(ax[e].base - 1 + k) * S.h - xc[e];
+  }
+  // columns of Q: (Q d)_r = sum_c Q.c[c][r] d_c
+  const F3 q0 = f3(Q.c[0][0], Q.c[0][1], Q.c[0][2]), q1 = f3(Q.c[1][0], Q.c[1][1], Q.c[1][2]),
+           q2 = f3(Q.c[2][0], Q.c[2][1], Q.c[2][2]);
+  F3 G0 = f3(0.f, 0.f, 0.f), Gx = G0, Gy = G0, Gz = G0;  // sum w gP, and times d_x, d_y, d_z
+  float SX = 0.f, SY = 0.f, SZ = 0.f;                     // sum dw s per axis
 #pragma unroll 1
-  for (int ia = 0; ia < 4; ++ia)
+  for (int ia = 0; ia < 4; ++ia) {
+    const F3 ua = mv + q0 * dd[0][ia];
+    F3 T0 = f3(0.f, 0.f, 0.f), Ty = T0, Tz = T0;
+    float Q0 = 0.f, Q1 = 0.f, Q2 = 0.f;
 #pragma unroll
     for (int ib = 0; ib < 4; ++ib) {
       __asm__ __volatile__("" ::: "memory");  // one z column of four node loads in flight
@@ -229,31 +249,53 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
       float4 gcol[4];
 #pragma unroll
       for (int ic = 0; ic < 4; ++ic) gcol[ic] = gnode[node_index(i, j, ax[2].base - 1 + ic, S)];
+      const F3 uab = ua + q1 * dd[1][ib];
+      F3 R0 = f3(0.f, 0.f, 0.f), Rz = R0;
+      float P0 = 0.f, P1 = 0.f;
 #pragma unroll
       for (int ic = 0; ic < 4; ++ic) {
-        const int k = ax[2].base - 1 + ic;
-        const float w = ax[0].w[ia] * ax[1].w[ib] * ax[2].w[ic];
-        const F3 dw = f3(ax[0].dw[ia] * ax[1].w[ib] * ax[2].w[ic], ax[0].w[ia] * ax[1].dw[ib] * ax[2].w[ic],
-                         ax[0].w[ia] * ax[1].w[ib] * ax[2].dw[ic]);
         const float4 gn = gcol[ic];
         const F3 gP = f3(gn.y, gn.z, gn.w);
-        const F3 d = f3((float)i * S.h - x.x, (float)j * S.h - x.y, (float)k * S.h - x.z);
-        const F3 Qd = mvec(Q, d);
-        const float s = m * gn.x + dot(gP, mv + Qd);
-        // Q^T gP
-        const F3 QtgP = f3(Q.c[0][0] * gP.x + Q.c[0][1] * gP.y + Q.c[0][2] * gP.z,
-                           Q.c[1][0] * gP.x + Q.c[1][1] * gP.y + Q.c[1][2] * gP.z,
-                           Q.c[2][0] * gP.x + Q.c[2][1] * gP.y + Q.c[2][2] * gP.z);
-        gx = gx + dw * s - QtgP * w;
-        gv = gv + gP * (w * m);
-        // gQ(r, c) += w gP_r d_c
-        const float dd[3] = {d.x, d.y, d.z}, gp[3] = {gP.x, gP.y, gP.z};
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int r = 0; r < 3; ++r) gQ.c[c][r] += w * gp[r] * dd[c];
+        const F3 u = uab + q2 * dd[2][ic];
+        const float sn = m * gn.x + dot(gP, u);
+        const float wz = ax[2].w[ic];
+        P0 = fmaf(wz, sn, P0);
+        P1 = fmaf(ax[2].dw[ic], sn, P1);
+        R0 = R0 + gP * wz;
+        Rz = Rz + gP * (wz * dd[2][ic]);
       }
+      const float wy = ax[1].w[ib];
+      Q0 = fmaf(wy, P0, Q0);
+      Q1 = fmaf(ax[1].dw[ib], P0, Q1);
+      Q2 = fmaf(wy, P1, Q2);
+      T0 = T0 + R0 * wy;
+      Ty = Ty + R0 * (wy * dd[1][ib]);
+      Tz = Tz + Rz * wy;
     }
+    const float wx = ax[0].w[ia];
+    SX = fmaf(ax[0].dw[ia], Q0, SX);
+    SY = fmaf(wx, Q1, SY);
+    SZ = fmaf(wx, Q2, SZ);
+    G0 = G0 + T0 * wx;
+    Gx = Gx + T0 * (wx * dd[0][ia]);
+    Gy = Gy + Ty * wx;
+    Gz = Gz + Tz * wx;
+  }
+  F3 gx = f3(fld(Gout, sd, AX, p), fld(Gout, sd, AX + 1, p), fld(Gout, sd, AX + 2, p));
+  // Q^T G0
+  const F3 QtG = f3(dot(q0, G0), dot(q1, G0), dot(q2, G0));
+  gx = gx + f3(SX, SY, SZ) - QtG;
+  F3 gv = G0 * m;
+  M3 gQ;
+  gQ.c[0][0] = Gx.x;
+  gQ.c[0][1] = Gx.y;
+  gQ.c[0][2] = Gx.z;
+  gQ.c[1][0] = Gy.x;
+  gQ.c[1][1] = Gy.y;
+  gQ.c[1][2] = Gy.z;
+  gQ.c[2][0] = Gz.x;
+  gQ.c[2][1] = Gz.y;
+  gQ.c[2][2] = Gz.z;
   __asm__ __volatile__("" ::: "memory");  // reload the rows below rather than keep them live
   // forward stress pieces, rebuilt
   const M3 B = ld_m3(St, sd, FB, p), FE = ld_m3(St, sd, FFE, p), FP = ld_m3(St, sd, FFP, p);
