@@ -630,33 +630,76 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
         return u * w;
       };
       if (fits) {
-        // the block pass's per-lane traversal order (lane_rotation)
+        // the block pass's per-lane traversal order (lane_rotation): step k
+        // of axis e visits node (k + r_e) & 3, whose weights rot4 moves to
+        // index k.  The position adjoint is separable but for
+        // s_i = v_i . u_i: gx = sum dw_i s_i - gB^T sum w_i v_i, summed over
+        // z, then y, then x (about 21 vector instructions per node instead
+        // of 50, the three scatter atomics included).
         const int l_ = t & 63;
-        const int r0 = ((l_ >> 4) - lb[0]) & 3, r1 = (((l_ >> 2) & 3) - lb[1]) & 3, r2 = ((l_ & 3) - lb[2]) & 3;
-        rot4(ax[0].w, r0);
-        rot4(ax[0].dw, r0);
-        rot4(ax[1].w, r1);
-        rot4(ax[1].dw, r1);
-        rot4(ax[2].w, r2);
-        rot4(ax[2].dw, r2);
+        const int rr[3] = {((l_ >> 4) - lb[0]) & 3, (((l_ >> 2) & 3) - lb[1]) & 3, ((l_ & 3) - lb[2]) & 3};
+        float ddr[3][4];
+        {
+          const float xc[3] = {x.x, x.y, x.z};
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            rot4(ax[e].w, rr[e]);
+            rot4(ax[e].dw, rr[e]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ddr[e][k] = (float)(ax[e].base - 1 + ((k + rr[e]) & 3)) * S.h - xc[e];
+          }
+        }
+        // columns of gB: (gB d)_r = sum_c gB.c[c][r] d_c
+        const F3 b0 = f3(gB.c[0][0], gB.c[0][1], gB.c[0][2]), b1 = f3(gB.c[1][0], gB.c[1][1], gB.c[1][2]),
+                 b2 = f3(gB.c[2][0], gB.c[2][1], gB.c[2][2]);
+        const F3 gv3 = f3(gvv[0], gvv[1], gvv[2]);
+        float SXs = 0.f, SYs = 0.f, SZs = 0.f;
+        F3 V = f3(0.f, 0.f, 0.f);  // sum w_i v_i
 #pragma unroll
         for (int ia = 0; ia < 4; ++ia) {
           __asm__ __volatile__("" ::: "memory");  // at most one x slab of nodes in flight
-          const int ja = (ia + r0) & 3;
+          const int ja = (ia + rr[0]) & 3;
+          const F3 ua = gv3 + b0 * ddr[0][ia];
+          const float wa = ax[0].w[ia];
+          float Q0 = 0.f, Q1 = 0.f, Q2 = 0.f;
+          F3 Tv = f3(0.f, 0.f, 0.f);
 #pragma unroll
           for (int ib = 0; ib < 4; ++ib) {
-            const int jb = (ib + r1) & 3;
+            const int jb = (ib + rr[1]) & 3;
+            const F3 uab = ua + b1 * ddr[1][ib];
+            const float wab = wa * ax[1].w[ib];
+            float P0 = 0.f, P1 = 0.f;
+            F3 Rv = f3(0.f, 0.f, 0.f);
 #pragma unroll
             for (int ic = 0; ic < 4; ++ic) {
-              const int jc = (ic + r2) & 3;
+              const int jc = (ic + rr[2]) & 3;
               const int q = q0 + ja * SX + jb * SY + jc;
-              const F3 wu = node(ia, ib, ic, ja, jb, jc, f3(vh[q], vh[PLANE + q], vh[2 * PLANE + q]));
-              atomicAdd(&gh[q], (double)wu.x);
-              atomicAdd(&gh[PLANE + q], (double)wu.y);
-              atomicAdd(&gh[2 * PLANE + q], (double)wu.z);
+              const F3 vi = f3(vh[q], vh[PLANE + q], vh[2 * PLANE + q]);
+              const F3 u = uab + b2 * ddr[2][ic];
+              const float su = dot(vi, u);
+              const float wz = ax[2].w[ic];
+              P0 = fmaf(wz, su, P0);
+              P1 = fmaf(ax[2].dw[ic], su, P1);
+              Rv = Rv + vi * wz;
+              const float w = wab * wz;
+              atomicAdd(&gh[q], (double)(u.x * w));
+              atomicAdd(&gh[PLANE + q], (double)(u.y * w));
+              atomicAdd(&gh[2 * PLANE + q], (double)(u.z * w));
             }
+            const float wy = ax[1].w[ib];
+            Q0 = fmaf(wy, P0, Q0);
+            Q1 = fmaf(ax[1].dw[ib], P0, Q1);
+            Q2 = fmaf(wy, P1, Q2);
+            Tv = Tv + Rv * wy;
           }
+          SXs = fmaf(ax[0].dw[ia], Q0, SXs);
+          SYs = fmaf(wa, Q1, SYs);
+          SZs = fmaf(wa, Q2, SZs);
+          V = V + Tv * wa;
         }
+        gx[0] += SXs - dot(b0, V);
+        gx[1] += SYs - dot(b1, V);
+        gx[2] += SZs - dot(b2, V);
       } else {
         for (int ia = 0; ia < 4; ++ia)
           for (int ib = 0; ib < 4; ++ib)
