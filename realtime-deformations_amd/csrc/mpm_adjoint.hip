@@ -170,10 +170,34 @@ __global__ void __launch_bounds__(TPB) k_adj_map(GridRec r, int set, float4* gno
   }
 }
 
+// The SVD F = Lb diag(s) Rb^T from the 21 rows f_update_t recorded for F^T
+// (U, S, V: Lb = V, Rb = U), singular values clamped to [lo, hi] when the
+// rows describe the update that produced F (F_E' = V clamp(S) U^T).
+__device__ __forceinline__ void svd_from_rows(const float* Sv, int64_t sd, int p, bool clamp, float lo, float hi,
+                                              M3& Lb, float (&s)[3], M3& Rb) {
+  float U[3][3], V[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float si = Sv[(int64_t)(9 + i) * sd + p];
+    s[i] = clamp ? fminf(fmaxf(si, lo), hi) : si;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      U[i][j] = Sv[(int64_t)(3 * i + j) * sd + p];
+      V[i][j] = Sv[(int64_t)(12 + 3 * i + j) * sd + p];
+    }
+  }
+  Lb = from_rows(V);
+  Rb = from_rows(U);
+}
+
 // P2G, stress, hardening and F-update adjoint for substep n.  St: state of
 // substep n; Gin: adjoints of substep n+1 (F_E, F_P); Gout: on entry the
 // partial position adjoint, on exit all adjoints of substep n; gE += dL/dE0.
-__global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE) {
+// Sv: the SVD the forward's F update of substep n used; SvPrev: that of
+// substep n - 1, whose clamped singular values and vectors are the SVD of
+// F_E of substep n (null at a segment's first substep: computed here).
+__global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE,
+                                                 const float* Sv, const float* SvPrev) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.n) return;
   const Scene S = a.scenes[scene_of_slot(a.scenes, a.nscenes, p)];
@@ -197,7 +221,11 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
     const float V0 = fld(a.P, sd, FVOL, p), E = fld(a.P, sd, FE0, p);
     const float hard = expf(xi * (1.0f - mdet(FP)));
     const float mu = E / S.lame.dmu * hard, lam = (E * nu) / S.lame.dlam * hard;
-    svd_of(FE, Lb, sv, Rb);
+    // the previous update's SVD is F_E's unless an FP64 re-solve replaced it
+    if (SvPrev && S.gap64 <= 0.0f)
+      svd_from_rows(SvPrev, sd, p, true, S.lo, S.hi, Lb, sv, Rb);
+    else
+      svd_of(FE, Lb, sv, Rb);
     const M3 R = mmul(Lb, mtrans(Rb));
     const float J = mdet(FE);
     const M3 K = mtrans(minverse(FE));  // F_E^-T
@@ -339,7 +367,7 @@ __global__ void __launch_bounds__(TPB) k_adj_p2g(KArgs a, const float* St, const
     const M3 Ftot = mmul(Fh, FP);
     M3 Lh, Rh;
     float sh[3];
-    svd_of(Fh, Lh, sh, Rh);
+    svd_from_rows(Sv, sd, p, false, 0.0f, 0.0f, Lh, sh, Rh);
     float f[3], fp[3];
     for (int i = 0; i < 3; ++i) {
       f[i] = fminf(fmaxf(sh[i], S.lo), S.hi);
@@ -425,8 +453,8 @@ void launch_adj_map(const KArgs& a, const GridRec& r, int set, float4* gnode, hi
   k_adj_map<<<rec_grid(a), TPB, 0, s>>>(r, set, gnode);
 }
 void launch_adj_p2g(const KArgs& a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE,
-                    hipStream_t s) {
-  if (a.n) k_adj_p2g<<<nblk(a.n), TPB, 0, s>>>(a, St, Gin, Gout, gnode, gE);
+                    const float* Sv, const float* SvPrev, hipStream_t s) {
+  if (a.n) k_adj_p2g<<<nblk(a.n), TPB, 0, s>>>(a, St, Gin, Gout, gnode, gE, Sv, SvPrev);
 }
 void launch_rows_to_slots(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows,
                           hipStream_t s) {

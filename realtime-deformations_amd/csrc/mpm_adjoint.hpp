@@ -13,7 +13,7 @@ void launch_adj_grid(const KArgs& a, const GridRec& r, float4* gnode, hipStream_
 // adjoints zeroed (set = 0)
 void launch_adj_map(const KArgs& a, const GridRec& r, int set, float4* gnode, hipStream_t s);
 void launch_adj_p2g(const KArgs& a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE,
-                    hipStream_t s);
+                    const float* Sv, const float* SvPrev, hipStream_t s);
 // row transfers: slot order (pid: caller index per slot) <-> caller order
 void launch_rows_to_slots(float* dst, const float* src, const int* pid, int64_t stride, int n, int rows,
                           hipStream_t s);

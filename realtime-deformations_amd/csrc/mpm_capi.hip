@@ -457,12 +457,12 @@ int mpm_destroy(mpm_ctx* ctx) {
   (void)hipFree(ctx->adj_const);
   (void)hipFree(ctx->adj_inv);
   for (auto& g : ctx->adj_segs) {
-    void* b[] = {g.st, g.gs, g.gl, g.pid, g.blist, g.nblk};
+    void* b[] = {g.st, g.gs, g.gl, g.sv, g.pid, g.blist, g.nblk};
     for (void* q : b) (void)hipFree(q);
   }
   {
     const mpm_ctx::AdjWork& w = ctx->adj_ws;
-    void* b[] = {w.Gc, w.Gs, w.Gs2, w.gEs, w.gEc, w.tmp, w.gnode, w.gmap, w.seg, w.gsave, w.glist};
+    void* b[] = {w.Gc, w.Gs, w.Gs2, w.gEs, w.gEc, w.tmp, w.gnode, w.gmap, w.seg, w.gsave, w.glist, w.svd};
     for (void* q : b) (void)hipFree(q);
   }
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
@@ -631,6 +631,7 @@ BlockArgs block_args(mpm_ctx* ctx, float dt) {
   ba.nblocks = ctx->nblocks;
   ba.grid = ctx->block_grid;
   ba.rec = nullptr;
+  ba.rec_svd = nullptr;
   ba.grec = GridRec{nullptr, nullptr, nullptr};
   ba.bctr = ctx->bctr;
   return ba;
@@ -1123,16 +1124,18 @@ GridRec grid_record(const mpm_ctx* ctx, float4* gs, int* gl, int* map, int k) {
 // Run ns substeps from the current binning on the fused path (the first one
 // a P2G pass, then fused passes, then a final G2P pass: mpm_substeps(dt, ns)
 // without re-binning), the passes writing each substep's starting state
-// (rows 0..32 in slot order, seg + k * 33 rows) and the positions after the
-// last one (seg + ns * 33 rows), and the grid pass recording the P2G sums of
-// each substep's active grid blocks (grid_record(k)).
-int record_segment(mpm_ctx* ctx, float dt, int ns, float* seg, float4* gs, int* gl) {
+// (rows 0..32 in slot order, seg + k * 33 rows), the SVD its F update used
+// (sv + k * SVD_ROWS rows) and the positions after the last one (seg + ns *
+// 33 rows), and the grid pass recording the P2G sums of each substep's
+// active grid blocks (grid_record(k)).
+int record_segment(mpm_ctx* ctx, float dt, int ns, float* seg, float4* gs, int* gl, float* sv) {
   const int64_t sd = ctx->stride;
   hipStream_t s = ctx->stream;
   if (!ctx->gacc_clean) reset_grid(ctx);
   for (int k = 0; k < ns; ++k) {
     BlockArgs ba = block_args(ctx, dt);
     ba.rec = seg + (size_t)k * NA * sd;
+    ba.rec_svd = sv + (size_t)k * SVD_ROWS * sd;
     ba.grec = grid_record(ctx, gs, gl, nullptr, k);
     launch_block_pass(ba, k == 0 ? PASS_P2G : PASS_FUSED, s);
     launch_grid_active(ba, s);
@@ -1148,7 +1151,7 @@ int record_segment(mpm_ctx* ctx, float dt, int ns, float* seg, float4* gs, int* 
 
 void free_segments(mpm_ctx* ctx) {
   for (auto& g : ctx->adj_segs) {
-    void* b[] = {g.st, g.gs, g.gl, g.pid, g.blist, g.nblk};
+    void* b[] = {g.st, g.gs, g.gl, g.sv, g.pid, g.blist, g.nblk};
     for (void* q : b) (void)hipFree(q);
   }
   ctx->adj_segs.clear();
@@ -1180,15 +1183,17 @@ int adjoint_workspace(mpm_ctx* ctx, int every) {
   if (every > 1 && every > w.seg_every) {
     // states of a segment's substeps plus the positions after it; the grid
     // records of each substep
-    void* b[] = {w.seg, w.gsave, w.glist};
+    void* b[] = {w.seg, w.gsave, w.glist, w.svd};
     for (void* q : b) (void)hipFree(q);
     w.seg = nullptr;
     w.gsave = nullptr;
     w.glist = nullptr;
+    w.svd = nullptr;
     w.seg_every = 0;
     alloc(&w.seg, rows * (size_t)(every + 1));
     alloc(&w.gsave, sizeof(float4) * (size_t)ctx->nnodes * every);
     alloc(&w.glist, sizeof(int) * (size_t)(ctx->nblocks + 1) * every);
+    alloc(&w.svd, sizeof(float) * SVD_ROWS * sd * (size_t)every);
     if (ok) w.seg_every = every;
   }
   if (!ok) {
@@ -1200,8 +1205,8 @@ int adjoint_workspace(mpm_ctx* ctx, int every) {
 
 // gnode (zero on entry and exit outside the blocks being reversed) holds the
 // node adjoints; map: one int per grid block, -1 on entry and exit.
-void adjoint_segment(mpm_ctx* ctx, float dt, int ns, const float* seg, float4* gs, int* gl, int* map, float*& Gs,
-                     float*& Gs2, float4* gnode, float* gEs) {
+void adjoint_segment(mpm_ctx* ctx, float dt, int ns, const float* seg, float4* gs, int* gl, const float* sv, int* map,
+                     float*& Gs, float*& Gs2, float4* gnode, float* gEs) {
   const int64_t sd = ctx->stride;
   hipStream_t s = ctx->stream;
   for (int k = ns - 1; k >= 0; --k) {
@@ -1212,7 +1217,8 @@ void adjoint_segment(mpm_ctx* ctx, float dt, int ns, const float* seg, float4* g
     launch_adj_map(ba.a, ba.grec, 1, gnode, s);
     launch_adj_g2p_block(ba, St, Xn, Gs, Gs2, gnode, s);
     launch_adj_grid(ba.a, ba.grec, gnode, s);
-    launch_adj_p2g(ba.a, St, Gs, Gs2, gnode, gEs, s);
+    const float* Sv = sv + (size_t)k * SVD_ROWS * sd;
+    launch_adj_p2g(ba.a, St, Gs, Gs2, gnode, gEs, Sv, k > 0 ? Sv - (size_t)SVD_ROWS * sd : nullptr, s);
     launch_adj_map(ba.a, ba.grec, 0, gnode, s);
     std::swap(Gs, Gs2);
   }
@@ -1239,7 +1245,7 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
     // device memory (or are already held for this shape), else checkpoints
     // every 20 substeps
     size_t free_b = 0, total_b = 0;
-    const double per_step = (double)sizeof(float) * NA * sd + (double)sizeof(float4) * ctx->nnodes +
+    const double per_step = (double)sizeof(float) * (NA + SVD_ROWS) * sd + (double)sizeof(float4) * ctx->nnodes +
                             (double)sizeof(int) * (ctx->nblocks + 1);
     const double need = per_step * (nsteps + (double)seg_ns.size()) + (double)sizeof(float) * NA * sd;
     every = (same && !ctx->adj_segs.empty()) ||
@@ -1278,6 +1284,7 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
         bool ok = hipMalloc(&g.st, rows * (size_t)(g.ns + 1)) == hipSuccess &&
                   hipMalloc(&g.gs, sizeof(float4) * (size_t)ctx->nnodes * g.ns) == hipSuccess &&
                   hipMalloc(&g.gl, sizeof(int) * (size_t)(ctx->nblocks + 1) * g.ns) == hipSuccess &&
+                  hipMalloc(&g.sv, sizeof(float) * SVD_ROWS * sd * (size_t)g.ns) == hipSuccess &&
                   hipMalloc(&g.pid, sizeof(int) * (size_t)std::max(1, ctx->n)) == hipSuccess &&
                   hipMalloc(&g.blist, sizeof(int4) * (size_t)ctx->nblocks) == hipSuccess &&
                   hipMalloc(&g.nblk, sizeof(int)) == hipSuccess;
@@ -1293,7 +1300,7 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
       MPM_HIP(hipMemcpyAsync(g.pid, ctx->pid, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
       MPM_HIP(hipMemcpyAsync(g.blist, ctx->blist, sizeof(int4) * ctx->nblocks, hipMemcpyDeviceToDevice, ctx->stream));
       MPM_HIP(hipMemcpyAsync(g.nblk, ctx->nblk, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
-      if (int rc = record_segment(ctx, dt, g.ns, g.st, g.gs, g.gl)) return rc;
+      if (int rc = record_segment(ctx, dt, g.ns, g.st, g.gs, g.gl, g.sv)) return rc;
     }
     ctx->gacc_clean = false;
   }
@@ -1359,7 +1366,7 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
       launch_rows_to_slots(ctx->P + (int64_t)FM * sd, ctx->adj_const, ctx->pid, sd, n, 3, s);
       launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
       MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
-      adjoint_segment(ctx, dt, g.ns, g.st, g.gs, g.gl, gmap, Gs, Gs2, gnode, gEs);
+      adjoint_segment(ctx, dt, g.ns, g.st, g.gs, g.gl, g.sv, gmap, Gs, Gs2, gnode, gEs);
       launch_rows_to_caller(Gc, Gs, ctx->pid, ctx->adj_inv, sd, n, NA, s);
       launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
     }
@@ -1369,10 +1376,10 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
     const int ns = std::min(every, steps - c * every);
     if ((rc = restore_rows(ctx, ctx->adj_ckpt[c])) || (rc = rebin(ctx))) break;
     // re-run the segment, keeping each substep's starting state and P2G sums
-    if ((rc = record_segment(ctx, dt, ns, seg, gsave, glist))) break;
+    if ((rc = record_segment(ctx, dt, ns, seg, gsave, glist, w.svd))) break;
     launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
     MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
-    adjoint_segment(ctx, dt, ns, seg, gsave, glist, gmap, Gs, Gs2, gnode, gEs);
+    adjoint_segment(ctx, dt, ns, seg, gsave, glist, w.svd, gmap, Gs, Gs2, gnode, gEs);
     launch_rows_to_caller(Gc, Gs, ctx->pid, ctx->adj_inv, sd, n, NA, s);
     launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
     ctx->gacc_clean = false;

@@ -377,7 +377,12 @@ __device__ __forceinline__ void particle_update(const BlockArgs& ba, const Scene
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int r = 0; r < 3; ++r) Q.c[c][r] = fmaf(m, BD.c[c][r], -a.dt * A.c[c][r]);
-    if (f_update_t<COMPOSE>(a.dt, Dinv, S.lo, S.hi, S.gap64, B, FE, FP)) {
+    float svd[SVD_ROWS];
+    if (f_update_t<COMPOSE>(a.dt, Dinv, S.lo, S.hi, S.gap64, B, FE, FP, REC ? svd : nullptr)) {
+      if (REC) {
+#pragma unroll
+        for (int r = 0; r < SVD_ROWS; ++r) ba.rec_svd[(int64_t)r * a.stride + p] = svd[r];
+      }
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll

@@ -41,10 +41,14 @@ struct BlockArgs {
   // to rec, the G2P pass its new positions to rec's x rows; with grec.acc
   // set the grid pass records the substep's P2G sums there
   float* rec;
+  // with rec: the SVD the F update used (SVD_ROWS rows of f_update_t's svd
+  // output, slot order), for the reverse mode to read back
+  float* rec_svd;
   GridRec grec;
 };
 
 enum { PASS_P2G = 0, PASS_FUSED = 1, PASS_G2P = 2, PASS_MASS = 3 };
+constexpr int SVD_ROWS = 21;
 
 // Reverse mode's advection + G2P adjoint on the block structure: per block,
 // node velocities (from the substep's grid record ba.grec, its map set) in an

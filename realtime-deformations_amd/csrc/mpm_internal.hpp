@@ -218,6 +218,7 @@ struct mpm_ctx {
     float* st = nullptr;     // (ns + 1) x 33 rows
     float4* gs = nullptr;    // ns x nnodes: P2G sums of the active grid blocks
     int* gl = nullptr;       // ns x (nblocks + 1): their count and list
+    float* sv = nullptr;     // ns x 21 rows: the F updates' SVDs
     int* pid = nullptr;      // n
     int4* blist = nullptr;   // nblocks
     int* nblk = nullptr;     // 1
@@ -231,7 +232,8 @@ struct mpm_ctx {
     float* seg = nullptr;     // (every + 1) x 33 rows
     float4* gsave = nullptr;  // every x nnodes
     int* glist = nullptr;     // every x (nblocks + 1)
-    int seg_every = 0;        // substeps seg / gsave / glist hold
+    float* svd = nullptr;     // every x 21 rows: the F updates' SVDs
+    int seg_every = 0;        // substeps seg / gsave / glist / svd hold
   } adj_ws;
   // positions readback: device staging, pinned host mirror, copy stream
   float* pos_stage = nullptr;

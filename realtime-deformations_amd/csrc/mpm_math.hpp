@@ -531,9 +531,11 @@ MPM_HD_NOINLINE void textbook_fp64(const M3& FEhat, double lo, double hi, M3& FE
 // material_point_method.cpp:306-330 for one particle; Bn is the APIC matrix
 // of the previous G2P.  Returns false when the SVD rejects its input; the
 // particle's F is then left unchanged and the caller raises a status flag.
+// svd (optional): the 21 values of the SVD of F^T = U S V^T the update used
+// (U row-major, S unclamped, V row-major), which the reverse mode reads back.
 template <int COMPOSE>
 MPM_HD bool f_update_t(float dt, const M3& Dinv, float lo, float hi, float gap64, const M3& Bn,
-                                           M3& FE, M3& FP) {
+                                           M3& FE, M3& FP, float* svd = nullptr) {
   const int compose = COMPOSE;
   const M3 FPn1 = mmul(mmul(madd(mdiag(1.0f), mscale(mul_dinv(Bn, Dinv), dt)), FE), FP);
   const M3 FEhat = mmul(FPn1, minverse(FP));
@@ -543,6 +545,17 @@ MPM_HD bool f_update_t(float dt, const M3& Dinv, float lo, float hi, float gap64
 #pragma unroll
     for (int j = 0; j < 3; ++j) M[i][j] = FEhat.c[i][j];
   if (!jacobi_svd3<float>(M, U, S, V)) return false;
+  if (svd) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      svd[9 + i] = S[i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        svd[3 * i + j] = U[i][j];
+        svd[12 + 3 * i + j] = V[i][j];
+      }
+    }
+  }
   M3 FEn;
   const bool fp64 = COMPOSE == COMPOSE_TEXTBOOK && gap64 > 0.0f &&
                     fminf(S[0] - S[1], S[1] - S[2]) < gap64 * S[0];
