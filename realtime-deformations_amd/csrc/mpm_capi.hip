@@ -1134,6 +1134,9 @@ int record_segment(mpm_ctx* ctx, float dt, int ns, float* seg, float4* gs, int* 
   if (!ctx->gacc_clean) reset_grid(ctx);
   for (int k = 0; k < ns; ++k) {
     BlockArgs ba = block_args(ctx, dt);
+    // the recording passes sum in the default (FP32) format, so the grid
+    // pass must read that format in reproducible mode too
+    ba.a.exact = 0;
     ba.rec = seg + (size_t)k * NA * sd;
     ba.rec_svd = sv + (size_t)k * SVD_ROWS * sd;
     ba.grec = grid_record(ctx, gs, gl, nullptr, k);
@@ -1143,6 +1146,7 @@ int record_segment(mpm_ctx* ctx, float dt, int ns, float* seg, float4* gs, int* 
     ++ctx->since_sort;
   }
   BlockArgs ba = block_args(ctx, dt);
+  ba.a.exact = 0;
   ba.rec = seg + (size_t)ns * NA * sd;
   launch_block_pass(ba, PASS_G2P, s);
   MPM_HIP(hipGetLastError());
