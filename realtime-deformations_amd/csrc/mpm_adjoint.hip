@@ -158,15 +158,38 @@ __global__ void __launch_bounds__(TPB) k_adj_grid(KArgs a, GridRec r, float4* gn
   }
 }
 
-// set the record's block map (set = 1) before the substep is reversed, or
-// after it reset the map to -1 and zero the node adjoints of its blocks
-__global__ void __launch_bounds__(TPB) k_adj_map(GridRec r, int set, float4* gnode) {
+// set the record's block map (set = 1) before the substep is reversed and
+// write the node velocities of its blocks, v = walls(P / m + dt g) as the
+// grid pass computed them, to vnode (read by the reverse G2P); or after it
+// (set = 0) reset the map to -1 and zero the node adjoints and velocities of
+// its blocks, so both stay zero outside the blocks being reversed
+__global__ void __launch_bounds__(TPB) k_adj_map(KArgs a, GridRec r, int set, float4* gnode, float4* vnode) {
   const int lane = threadIdx.x & 63;
   const int cnt = r.list[0];
   for (int s = blockIdx.x * 4 + (threadIdx.x >> 6); s < cnt; s += gridDim.x * 4) {
     const int gb = r.list[1 + s];
+    const int64_t g = (int64_t)gb * 64 + lane;
     if (lane == 0) r.map[gb] = set ? s : -1;
-    if (!set) gnode[(int64_t)gb * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!set) {
+      gnode[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+      vnode[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
+    }
+    const float4 acc = r.acc[(int64_t)s * 64 + lane];
+    F3 v = f3(0.f, 0.f, 0.f);
+    if (acc.x != 0.0f) {
+      const Scene S = a.scenes[a.blk_scene[gb]];
+      v = f3(acc.y / acc.x + a.dt * S.g[0], acc.z / acc.x + a.dt * S.g[1], acc.w / acc.x + a.dt * S.g[2]);
+      if (S.bc_mode & MPM_BC_DOMAIN) {
+        const int64_t lb = gb - S.blk0;
+        const int bk = (int)(lb % S.nb[2]);
+        const int64_t t2 = lb / S.nb[2];
+        const int bj = (int)(t2 % S.nb[1]), bi = (int)(t2 / S.nb[1]);
+        v = domain_walls(bi * 4 + (lane >> 4), bj * 4 + ((lane >> 2) & 3), bk * 4 + (lane & 3), S.dims[0], S.dims[1],
+                         S.dims[2], v);
+      }
+    }
+    vnode[g] = make_float4(v.x, v.y, v.z, 0.f);
   }
 }
 
@@ -449,8 +472,8 @@ inline int rec_grid(const KArgs& a) { return (int)std::max<int64_t>(1, std::min<
 void launch_adj_grid(const KArgs& a, const GridRec& r, float4* gnode, hipStream_t s) {
   k_adj_grid<<<rec_grid(a), TPB, 0, s>>>(a, r, gnode);
 }
-void launch_adj_map(const KArgs& a, const GridRec& r, int set, float4* gnode, hipStream_t s) {
-  k_adj_map<<<rec_grid(a), TPB, 0, s>>>(r, set, gnode);
+void launch_adj_map(const KArgs& a, const GridRec& r, int set, float4* gnode, float4* vnode, hipStream_t s) {
+  k_adj_map<<<rec_grid(a), TPB, 0, s>>>(a, r, set, gnode, vnode);
 }
 void launch_adj_p2g(const KArgs& a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE,
                     const float* Sv, const float* SvPrev, hipStream_t s) {

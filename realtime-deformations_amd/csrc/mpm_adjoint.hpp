@@ -11,7 +11,7 @@ enum AdjField : int { AX = FX, AV = FV, AB = FB, AFE = FFE, AFP = FFP, NA = FM }
 void launch_adj_grid(const KArgs& a, const GridRec& r, float4* gnode, hipStream_t s);
 // r.map from r.list (set = 1); or back to -1 with the listed blocks' node
 // adjoints zeroed (set = 0)
-void launch_adj_map(const KArgs& a, const GridRec& r, int set, float4* gnode, hipStream_t s);
+void launch_adj_map(const KArgs& a, const GridRec& r, int set, float4* gnode, float4* vnode, hipStream_t s);
 void launch_adj_p2g(const KArgs& a, const float* St, const float* Gin, float* Gout, const float4* gnode, float* gE,
                     const float* Sv, const float* SvPrev, hipStream_t s);
 // row transfers: slot order (pid: caller index per slot) <-> caller order

@@ -462,7 +462,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   }
   {
     const mpm_ctx::AdjWork& w = ctx->adj_ws;
-    void* b[] = {w.Gc, w.Gs, w.Gs2, w.gEs, w.gEc, w.tmp, w.gnode, w.gmap, w.seg, w.gsave, w.glist, w.svd};
+    void* b[] = {w.Gc, w.Gs, w.Gs2, w.gEs, w.gEc, w.tmp, w.gnode, w.vnode, w.gmap, w.seg, w.gsave, w.glist, w.svd};
     for (void* q : b) (void)hipFree(q);
   }
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
@@ -1183,6 +1183,7 @@ int adjoint_workspace(mpm_ctx* ctx, int every) {
   alloc(&w.gEc, sizeof(float) * sd);
   alloc(&w.tmp, sizeof(float) * 3 * (size_t)std::max(1, ctx->n));
   alloc(&w.gnode, sizeof(float4) * (size_t)ctx->nnodes);
+  alloc(&w.vnode, sizeof(float4) * (size_t)ctx->nnodes);
   alloc(&w.gmap, sizeof(int) * (size_t)ctx->nblocks);
   if (every > 1 && every > w.seg_every) {
     // states of a segment's substeps plus the positions after it; the grid
@@ -1210,7 +1211,7 @@ int adjoint_workspace(mpm_ctx* ctx, int every) {
 // gnode (zero on entry and exit outside the blocks being reversed) holds the
 // node adjoints; map: one int per grid block, -1 on entry and exit.
 void adjoint_segment(mpm_ctx* ctx, float dt, int ns, const float* seg, float4* gs, int* gl, const float* sv, int* map,
-                     float*& Gs, float*& Gs2, float4* gnode, float* gEs) {
+                     float*& Gs, float*& Gs2, float4* gnode, float4* vnode, float* gEs) {
   const int64_t sd = ctx->stride;
   hipStream_t s = ctx->stream;
   for (int k = ns - 1; k >= 0; --k) {
@@ -1218,12 +1219,12 @@ void adjoint_segment(mpm_ctx* ctx, float dt, int ns, const float* seg, float4* g
     const float* Xn = seg + (size_t)(k + 1) * NA * sd;
     BlockArgs ba = block_args(ctx, dt);
     ba.grec = grid_record(ctx, gs, gl, map, k);
-    launch_adj_map(ba.a, ba.grec, 1, gnode, s);
-    launch_adj_g2p_block(ba, St, Xn, Gs, Gs2, gnode, s);
+    launch_adj_map(ba.a, ba.grec, 1, gnode, vnode, s);
+    launch_adj_g2p_block(ba, St, Xn, Gs, Gs2, gnode, vnode, s);
     launch_adj_grid(ba.a, ba.grec, gnode, s);
     const float* Sv = sv + (size_t)k * SVD_ROWS * sd;
     launch_adj_p2g(ba.a, St, Gs, Gs2, gnode, gEs, Sv, k > 0 ? Sv - (size_t)SVD_ROWS * sd : nullptr, s);
-    launch_adj_map(ba.a, ba.grec, 0, gnode, s);
+    launch_adj_map(ba.a, ba.grec, 0, gnode, vnode, s);
     std::swap(Gs, Gs2);
   }
 }
@@ -1338,12 +1339,13 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   float *const Gc = w.Gc, *const gEs = w.gEs, *const gEc = w.gEc, *const seg = w.seg, *const tmp = w.tmp;
   float*& Gs = w.Gs;  // Gs and Gs2 swap per substep
   float*& Gs2 = w.Gs2;
-  float4 *const gnode = w.gnode, *const gsave = w.gsave;
+  float4 *const gnode = w.gnode, *const vnode = w.vnode, *const gsave = w.gsave;
   int *const glist = w.glist, *const gmap = w.gmap;
   int rc = MPM_OK;
   MPM_HIP(hipMemsetAsync(Gc, 0, rows, s));
   MPM_HIP(hipMemsetAsync(gEc, 0, sizeof(float) * sd, s));
   launch_clear(gnode, ctx->nnodes, s);
+  launch_clear(vnode, ctx->nnodes, s);
   launch_fill_int(gmap, ctx->nblocks, -1, s);
   if (gx) {
     MPM_HIP(hipMemcpyAsync(tmp, gx, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
@@ -1370,7 +1372,7 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
       launch_rows_to_slots(ctx->P + (int64_t)FM * sd, ctx->adj_const, ctx->pid, sd, n, 3, s);
       launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
       MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
-      adjoint_segment(ctx, dt, g.ns, g.st, g.gs, g.gl, g.sv, gmap, Gs, Gs2, gnode, gEs);
+      adjoint_segment(ctx, dt, g.ns, g.st, g.gs, g.gl, g.sv, gmap, Gs, Gs2, gnode, vnode, gEs);
       launch_rows_to_caller(Gc, Gs, ctx->pid, ctx->adj_inv, sd, n, NA, s);
       launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
     }
@@ -1383,7 +1385,7 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
     if ((rc = record_segment(ctx, dt, ns, seg, gsave, glist, w.svd))) break;
     launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
     MPM_HIP(hipMemsetAsync(gEs, 0, sizeof(float) * sd, s));
-    adjoint_segment(ctx, dt, ns, seg, gsave, glist, w.svd, gmap, Gs, Gs2, gnode, gEs);
+    adjoint_segment(ctx, dt, ns, seg, gsave, glist, w.svd, gmap, Gs, Gs2, gnode, vnode, gEs);
     launch_rows_to_caller(Gc, Gs, ctx->pid, ctx->adj_inv, sd, n, NA, s);
     launch_add_to_caller(gEc, gEs, ctx->pid, n, s);
     ctx->gacc_clean = false;

@@ -36,6 +36,10 @@ using blk::HN;
 using blk::SX;
 using blk::SY;
 using blk::PLANE;
+using blk::TPLANE;
+using blk::TX;
+using blk::tile_y;
+using blk::tile_z;
 using blk::claim_block;
 
 enum Mode { MODE_P2G = 0, MODE_FUSED = 1, MODE_G2P = 2, MODE_MASS = 3 };
@@ -269,26 +273,26 @@ __device__ __forceinline__ void scatter_one(const BlockArgs& ba, const Scene& S,
     int qx[4], qy[4], qz[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      qx[k] = (s.lb[0] + ((k + r[0]) & 3)) * SX;
-      qy[k] = (s.lb[1] + ((k + r[1]) & 3)) * SY;
-      qz[k] = s.lb[2] + ((k + r[2]) & 3);
+      qx[k] = (s.lb[0] + ((k + r[0]) & 3)) * TX;
+      qy[k] = tile_y(s.lb[1] + ((k + r[1]) & 3));
+      qz[k] = tile_z(s.lb[2] + ((k + r[2]) & 3));
     }
     if (EXACT) {
       const double Mm = quantum_magic(S.qexp), Mp = quantum_magic(S.qexp + 2);
       p2g_scatter(S, s, r, x, m, mv, Q, [&](int ia, int ib, int ic, int, int, int, float vm, float px, float py, float pz) {
         const int q = qx[ia] + qy[ib] + qz[ic];
         atomicAdd(&ah[q], on_quantum((double)vm, Mm));
-        atomicAdd(&ah[PLANE + q], on_quantum((double)px, Mp));
-        atomicAdd(&ah[2 * PLANE + q], on_quantum((double)py, Mp));
-        atomicAdd(&ah[3 * PLANE + q], on_quantum((double)pz, Mp));
+        atomicAdd(&ah[TPLANE + q], on_quantum((double)px, Mp));
+        atomicAdd(&ah[2 * TPLANE + q], on_quantum((double)py, Mp));
+        atomicAdd(&ah[3 * TPLANE + q], on_quantum((double)pz, Mp));
       });
     } else {
       p2g_scatter(S, s, r, x, m, mv, Q, [&](int ia, int ib, int ic, int, int, int, float vm, float px, float py, float pz) {
         const int q = qx[ia] + qy[ib] + qz[ic];
         atomicAdd(&ah[q], (double)vm);
-        atomicAdd(&ah[PLANE + q], (double)px);
-        atomicAdd(&ah[2 * PLANE + q], (double)py);
-        atomicAdd(&ah[3 * PLANE + q], (double)pz);
+        atomicAdd(&ah[TPLANE + q], (double)px);
+        atomicAdd(&ah[2 * TPLANE + q], (double)py);
+        atomicAdd(&ah[3 * TPLANE + q], (double)pz);
       });
     }
   } else {
@@ -428,7 +432,7 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 2) k_blo
   constexpr bool kG2P = MODE == MODE_FUSED || MODE == MODE_G2P;
   constexpr bool kP2G = MODE != MODE_G2P;
   __shared__ float vh[kG2P ? 3 * PLANE : 1];
-  __shared__ double ah[kP2G ? 4 * PLANE : 1];
+  __shared__ double ah[kP2G ? 4 * TPLANE : 1];
   __shared__ int claimed;
   const KArgs& a = ba.a;
   const int t = threadIdx.x;
@@ -458,7 +462,7 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 2) k_blo
       }
     }
     if (kP2G) {
-      for (int q = t; q < 4 * PLANE; q += TPB) ah[q] = 0.0;
+      for (int q = t; q < 4 * TPLANE; q += TPB) ah[q] = 0.0;
     }
     __syncthreads();
     for (int p = start + t; p < end; p += TPB) {
@@ -503,9 +507,9 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 2) k_blo
     if (kP2G) {
       for (int q = t; q < HN * HN * HN; q += TPB) {
         const int hx = q / (HN * HN), hy = (q / HN) % HN, hz = q % HN;
-        const int l = hx * SX + hy * SY + hz;
+        const int l = hx * TX + tile_y(hy) + tile_z(hz);
         if (EXACT) {  // exact FP64 sums of multiples of the quanta
-          const double m = ah[l], px = ah[PLANE + l], py = ah[2 * PLANE + l], pz = ah[3 * PLANE + l];
+          const double m = ah[l], px = ah[TPLANE + l], py = ah[2 * TPLANE + l], pz = ah[3 * TPLANE + l];
           if (m != 0.0 || px != 0.0 || py != 0.0 || pz != 0.0) {
             const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
             if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2]) {
@@ -517,8 +521,8 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 2) k_blo
             }
           }
         } else {
-          const float m = (float)ah[l], px = (float)ah[PLANE + l], py = (float)ah[2 * PLANE + l],
-                      pz = (float)ah[3 * PLANE + l];
+          const float m = (float)ah[l], px = (float)ah[TPLANE + l], py = (float)ah[2 * TPLANE + l],
+                      pz = (float)ah[3 * TPLANE + l];
           if (m != 0.0f || px != 0.0f || py != 0.0f || pz != 0.0f) {
             const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
             if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2]) {
@@ -537,26 +541,19 @@ __global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 2) k_blo
   }
 }
 
-// grid velocity of a node from its P2G sums (m, P), as k_grid_active
-// computes it for a scene without colliders: v = P / m + dt g, domain walls
-__device__ __forceinline__ F3 node_velocity(float4 acc, const Scene& S, float dt, int i, int j, int k) {
-  if (acc.x == 0.0f) return f3(0.f, 0.f, 0.f);
-  F3 v = f3(acc.y / acc.x + dt * S.g[0], acc.z / acc.x + dt * S.g[1], acc.w / acc.x + dt * S.g[2]);
-  if (S.bc_mode & MPM_BC_DOMAIN) v = domain_walls(i, j, k, S.dims[0], S.dims[1], S.dims[2], v);
-  return v;
-}
-
 // Reverse of advection + G2P for the blocks of the current binning (see
-// launch_adj_g2p_block).  Node velocities come from the substep's saved P2G
-// sums (a.gacc); the advection clamp was active on an axis exactly when the
-// next stored position Xn sits on the clamp bound.  The stage order and
-// notation are those of mpm_adjoint.hip.
+// launch_adj_g2p_block).  Node velocities come from vnode, which the map
+// kernel filled from the substep's saved P2G sums (zero outside its listed
+// blocks); the advection clamp was active on an axis exactly when the next
+// stored position Xn sits on the clamp bound.  The stage order and notation
+// are those of mpm_adjoint.hip.
 template <int BATCHED>
 __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float* __restrict__ St,
                                                        const float* __restrict__ Xn, const float* __restrict__ Gin,
-                                                       float* __restrict__ Gout, float4* __restrict__ gnode) {
+                                                       float* __restrict__ Gout, float4* __restrict__ gnode,
+                                                       const float4* __restrict__ vnode) {
   __shared__ float vh[3 * PLANE];
-  __shared__ double gh[3 * PLANE];
+  __shared__ double gh[3 * TPLANE];  // tiled like the forward accumulators
   __shared__ int claimed;
   const KArgs& a = ba.a;
   const int t = threadIdx.x;
@@ -576,15 +573,15 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
     for (int q = t; q < HN * HN * HN; q += TPB) {
       const int hx = q / (HN * HN), hy = (q / HN) % HN, hz = q % HN;
       const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
-      F3 v = f3(0.f, 0.f, 0.f);
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
       if (gi >= 0 && gj >= 0 && gk >= 0 && gi < S.dims[0] && gj < S.dims[1] && gk < S.dims[2])
-        v = node_velocity(rec_acc(ba.grec, node_index(gi, gj, gk, S)), S, a.dt, gi, gj, gk);
+        v = vnode[node_index(gi, gj, gk, S)];
       const int l = hx * SX + hy * SY + hz;
       vh[l] = v.x;
       vh[PLANE + l] = v.y;
       vh[2 * PLANE + l] = v.z;
     }
-    for (int q = t; q < 3 * PLANE; q += TPB) gh[q] = 0.0;
+    for (int q = t; q < 3 * TPLANE; q += TPB) gh[q] = 0.0;
     __syncthreads();
     for (int p = start + t; p < end; p += TPB) {
       const F3 x = f3(St[(int64_t)FX * sd + p], St[(int64_t)(FX + 1) * sd + p], St[(int64_t)(FX + 2) * sd + p]);
@@ -658,6 +655,14 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
         const F3 b0 = f3(gB.c[0][0], gB.c[0][1], gB.c[0][2]), b1 = f3(gB.c[1][0], gB.c[1][1], gB.c[1][2]),
                  b2 = f3(gB.c[2][0], gB.c[2][1], gB.c[2][2]);
         const F3 gv3 = f3(gvv[0], gvv[1], gvv[2]);
+        // offsets of the tiled accumulator halo per traversal step and axis
+        int tqx[4], tqy[4], tqz[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          tqx[k] = (lb[0] + ((k + rr[0]) & 3)) * TX;
+          tqy[k] = tile_y(lb[1] + ((k + rr[1]) & 3));
+          tqz[k] = tile_z(lb[2] + ((k + rr[2]) & 3));
+        }
         float SXs = 0.f, SYs = 0.f, SZs = 0.f;
         F3 V = f3(0.f, 0.f, 0.f);  // sum w_i v_i
 #pragma unroll
@@ -687,9 +692,10 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
               P1 = fmaf(ax[2].dw[ic], su, P1);
               Rv = Rv + vi * wz;
               const float w = wab * wz;
-              atomicAdd(&gh[q], (double)(u.x * w));
-              atomicAdd(&gh[PLANE + q], (double)(u.y * w));
-              atomicAdd(&gh[2 * PLANE + q], (double)(u.z * w));
+              const int qg = tqx[ia] + tqy[ib] + tqz[ic];
+              atomicAdd(&gh[qg], (double)(u.x * w));
+              atomicAdd(&gh[TPLANE + qg], (double)(u.y * w));
+              atomicAdd(&gh[2 * TPLANE + qg], (double)(u.z * w));
             }
             const float wy = ax[1].w[ib];
             Q0 = fmaf(wy, P0, Q0);
@@ -712,8 +718,8 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
               const int i = ax[0].base - 1 + ia, j = ax[1].base - 1 + ib, k = ax[2].base - 1 + ic;
               const int64_t g = node_index(i, j, k, S);
               const int c = ba.grec.map[g >> 6];
-              const float4 acc = c < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : ba.grec.acc[(int64_t)c * 64 + (g & 63)];
-              const F3 wu = node(ia, ib, ic, ia, ib, ic, node_velocity(acc, S, a.dt, i, j, k));
+              const float4 vn = vnode[g];
+              const F3 wu = node(ia, ib, ic, ia, ib, ic, f3(vn.x, vn.y, vn.z));
               // massless nodes (outside the listed blocks) carry no adjoint
               if (c >= 0) {
                 float* gn = reinterpret_cast<float*>(gnode + g);
@@ -729,8 +735,8 @@ __global__ void __launch_bounds__(TPB) k_adj_g2p_block(BlockArgs ba, const float
     __syncthreads();
     for (int q = t; q < HN * HN * HN; q += TPB) {
       const int hx = q / (HN * HN), hy = (q / HN) % HN, hz = q % HN;
-      const int l = hx * SX + hy * SY + hz;
-      const float u0 = (float)gh[l], u1 = (float)gh[PLANE + l], u2 = (float)gh[2 * PLANE + l];
+      const int l = hx * TX + tile_y(hy) + tile_z(hz);
+      const float u0 = (float)gh[l], u1 = (float)gh[TPLANE + l], u2 = (float)gh[2 * TPLANE + l];
       if (u0 != 0.0f || u1 != 0.0f || u2 != 0.0f) {
         const int gi = o[0] + hx, gj = o[1] + hy, gk = o[2] + hz;
         const int64_t g = node_index(gi, gj, gk, S);
@@ -874,11 +880,11 @@ void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s) {
 }
 
 void launch_adj_g2p_block(const BlockArgs& ba, const float* St, const float* Xn, const float* Gin, float* Gout,
-                          float4* gnode, hipStream_t s) {
+                          float4* gnode, const float4* vnode, hipStream_t s) {
   if (ba.a.nscenes == 1)
-    k_adj_g2p_block<0><<<ba.grid, TPB, 0, s>>>(ba, St, Xn, Gin, Gout, gnode);
+    k_adj_g2p_block<0><<<ba.grid, TPB, 0, s>>>(ba, St, Xn, Gin, Gout, gnode, vnode);
   else
-    k_adj_g2p_block<1><<<ba.grid, TPB, 0, s>>>(ba, St, Xn, Gin, Gout, gnode);
+    k_adj_g2p_block<1><<<ba.grid, TPB, 0, s>>>(ba, St, Xn, Gin, Gout, gnode, vnode);
 }
 
 void launch_grid_active(const BlockArgs& ba, hipStream_t s) {

@@ -19,12 +19,6 @@ struct GridRec {
   int* map;
 };
 
-// P2G sums of node g from a grid record (zero outside the listed blocks)
-__device__ __forceinline__ float4 rec_acc(const GridRec& r, int64_t g) {
-  const int c = r.map[g >> 6];
-  return c < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : r.acc[(int64_t)c * 64 + (g & 63)];
-}
-
 struct BlockArgs {
   KArgs a;
   const int4* blist;  // (home block, start, end, 0)
@@ -51,14 +45,15 @@ enum { PASS_P2G = 0, PASS_FUSED = 1, PASS_G2P = 2, PASS_MASS = 3 };
 constexpr int SVD_ROWS = 21;
 
 // Reverse mode's advection + G2P adjoint on the block structure: per block,
-// node velocities (from the substep's grid record ba.grec, its map set) in an
+// node velocities (vnode, filled from the substep's grid record by the map
+// kernel; ba.grec's map set) in an
 // LDS halo and the node-velocity adjoints (w (gv' + gB' d)) accumulated in an
 // FP64 LDS halo, flushed with one set of global atomics per block into the
 // listed grid blocks.  St: stored state of the substep (slot order); Xn: rows
 // holding the next state's positions; Gin: the adjoints of the next state;
 // Gout.x receives the partial position adjoint; gnode.yzw the node adjoints.
 void launch_adj_g2p_block(const BlockArgs& ba, const float* St, const float* Xn, const float* Gin, float* Gout,
-                          float4* gnode, hipStream_t s);
+                          float4* gnode, const float4* vnode, hipStream_t s);
 void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s);
 void launch_grid_active(const BlockArgs& ba, hipStream_t s);
 void launch_advance_step(int* dstep, int* gcnt, hipStream_t s);

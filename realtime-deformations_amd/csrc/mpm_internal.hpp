@@ -228,6 +228,7 @@ struct mpm_ctx {
   struct AdjWork {
     float *Gc = nullptr, *Gs = nullptr, *Gs2 = nullptr, *gEs = nullptr, *gEc = nullptr, *tmp = nullptr;
     float4* gnode = nullptr;
+    float4* vnode = nullptr;  // node velocities of the substep being reversed
     int* gmap = nullptr;
     float* seg = nullptr;     // (every + 1) x 33 rows
     float4* gsave = nullptr;  // every x nnodes
