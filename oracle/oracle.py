@@ -279,9 +279,9 @@ def _block_layout(cells, srcs):
     """Slot order of one block: caller indices per slot.  cells: cell in the
     block (0..63) of the block's particles in stable sorted order, srcs: their
     caller indices.  Column g = cell & 15 (cy, cz); a column's list is
-    depth-major over its cells cx = cell >> 4 (depth d: the d-th particle of
-    the cell).  Slots come in groups of 16: every column with particles left
-    gives one, spare slots go one at a time to the column with the most left
+    cell-major: the particles of cell cx = cell >> 4 = 0 in sorted order,
+    then cx = 1, 2, 3.  Slots come in groups of 16: every column with
+    particles left gives one, spare slots go one at a time to the column with the most left
     (lowest column on ties), columns in order inside a group.  A block with a
     cell of more than COL_RR_MAX particles or more than COL_GMAX groups keeps
     the plain round-robin: the r-th particle of cell c at
@@ -301,7 +301,7 @@ def _block_layout(cells, srcs):
         return out
     col = []
     for g in range(16):
-        col.append([((cx << 4) | g, d) for d in range(maxc) for cx in range(4) if cnt[(cx << 4) | g] > d])
+        col.append([((cx << 4) | g, d) for cx in range(4) for d in range(cnt[(cx << 4) | g])])
     rem = [len(L) for L in col]
     ptr = [0] * 16
     out = []

@@ -292,45 +292,27 @@ __global__ void __launch_bounds__(256) k_block_ids(BinArgs b, const uint32_t* wg
   }
 }
 
-// Position of the k-th (0-based) set bit of m.
-__device__ __forceinline__ int select_bit(unsigned long long m, int k) {
-  int pos = 0;
-#pragma unroll
-  for (int w = 32; w >= 1; w >>= 1) {
-    const unsigned long long low = m & ((1ull << w) - 1ull);
-    const int c = __popcll(low);
-    if (k >= c) {
-      k -= c;
-      m >>= w;
-      pos += w;
-    } else {
-      m = low;
-    }
-  }
-  return pos;
-}
-
 // Slot layout inside a block, chosen for the LDS banks of the block pass.
 // A particle's stencil origin in the halo sets the FP64 bank pair of all its
 // atomics by its cell's column (cy, cz); a wave's 64 lanes issue an atomic as
 // four 16-lane groups, each free of conflicts when its 16 lanes come from 16
 // different columns.  The slots are therefore cut into groups of 16, and
-// each group takes one particle from every column that still has particles
-// (the column's list: depth-major over its four cells cx); when fewer than 16
-// columns remain, the group's spare slots go one by one to the column with
-// the most particles left (lowest column on ties).  Inside a group the
-// columns come in order.  A column's k-th taken particle is entry k of its
-// list; list entry p is the particle of depth d (the d-th of its cell in
-// sorted order) in the cell cx given by the depth's bit mask.
+// each group takes one particle from every column that still has particles;
+// when fewer than 16 columns remain, the group's spare slots go one by one
+// to the column with the most particles left (lowest column on ties).
+// Inside a group the columns come in order.  A column's k-th taken particle
+// is entry k of its list, which is cell-major: the particles of cell cx = 0
+// in sorted order, then cx = 1, 2, 3.  So the same lane of consecutive
+// groups mostly holds particles of one cell, and G2P reads of the two 16-lane
+// groups that issue together hit one address (a broadcast).
 //
-// Tables (one wave builds them): per column and depth the mask of cells with
-// more than d particles and the list position where depth d starts; per
-// group and column the first slot in the group and the list position at the
-// group's start.  Threads then walk destination slots (coalesced writes) and
-// find their sources by table lookups.  Blocks with a cell of more than
-// RR_MAX particles or more than GMAX groups take the plain round-robin of the
-// per-source formula instead (slot of the r-th particle of cell c:
-// sum_c' min(cnt[c'], r) + #{c' < c : cnt[c'] > r}).
+// Tables (one wave builds them): per group and column the first slot in the
+// group and the list position at the group's start.  Threads then walk
+// destination slots (coalesced writes) and find their sources by table
+// lookups.  Blocks with a cell of more than RR_MAX particles or more than
+// GMAX groups take the plain round-robin of the per-source formula instead
+// (slot of the r-th particle of cell c: sum_c' min(cnt[c'], r) +
+// #{c' < c : cnt[c'] > r}).
 constexpr int RR_MAX = 64;
 constexpr int GMAX = 256;
 __device__ __forceinline__ int wave_max(int v) {
@@ -361,11 +343,9 @@ constexpr int RO_T = 256;
 __global__ void __launch_bounds__(RO_T) k_reorder(BinArgs b) {
   __shared__ int cnt[64];
   __shared__ int cstart[64];
-  __shared__ unsigned char dmask[16][RR_MAX];
-  __shared__ unsigned short dbase[16][RR_MAX + 1];
   __shared__ unsigned char gofs[GMAX][16];
   __shared__ unsigned short gptr[GMAX][16];
-  __shared__ int maxc_s, table_s;
+  __shared__ int table_s;
   const int t = threadIdx.x;
   const int nblk = *b.nblk;
   for (int r = blockIdx.x; r < nblk; r += gridDim.x) {  // blocks in key order
@@ -387,26 +367,11 @@ __global__ void __launch_bounds__(RO_T) k_reorder(BinArgs b) {
       cstart[t] = incl - c;
       const int maxc = wave_max(c);
       const bool table = maxc <= RR_MAX && (nb + 15) / 16 <= GMAX;
-      if (t == 0) {
-        maxc_s = maxc;
-        table_s = table ? 1 : 0;
-      }
+      if (t == 0) table_s = table ? 1 : 0;
       if (table) {
-        // column lists (lanes 0..15: column g)
-        int ncol = 0;
-        if (t < 16) {
-          for (int d = 0; d < maxc; ++d) {
-            int m = 0;
-#pragma unroll
-            for (int cx = 0; cx < 4; ++cx) m |= (cnt[(cx << 4) | t] > d ? 1 : 0) << cx;
-            dmask[t][d] = (unsigned char)m;
-            dbase[t][d] = (unsigned short)ncol;
-            ncol += __popc(m);
-          }
-          dbase[t][maxc] = (unsigned short)ncol;
-        }
-        // groups of 16 slots (the whole wave runs the reductions; lanes >= 16 hold nothing)
-        int rem = t < 16 ? ncol : 0, ptr = 0, left = nb;
+        // groups of 16 slots (the whole wave runs the reductions; lanes 0..15
+        // hold column t, lanes >= 16 nothing)
+        int rem = t < 16 ? cnt[t] + cnt[16 | t] + cnt[32 | t] + cnt[48 | t] : 0, ptr = 0, left = nb;
         for (int gi = 0; left > 0; ++gi) {
           const int size = left < 16 ? left : 16;
           int take = rem > 0 ? 1 : 0;
@@ -434,21 +399,15 @@ __global__ void __launch_bounds__(RO_T) k_reorder(BinArgs b) {
     }
     __syncthreads();
     if (table_s) {
-      const int maxc = maxc_s;
       for (int dst = start + t; dst < end; dst += RO_T) {
         const int j = dst - start, gi = j >> 4, r = j & 15;
         int g = 15;
         while (gofs[gi][g] > r) --g;
         const int p = gptr[gi][g] + (r - gofs[gi][g]);
-        int lo = 0, hi = maxc - 1;  // last depth whose list start is <= p
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (dbase[g][mid] <= p) lo = mid;
-          else hi = mid - 1;
-        }
-        const int cx = select_bit((unsigned long long)dmask[g][lo], p - dbase[g][lo]);
+        int cx = 0, d = p;  // entry p of the cell-major list: particle d of cell cx
+        while (cx < 3 && d >= cnt[(cx << 4) | g]) d -= cnt[(cx++ << 4) | g];
         const int c = (cx << 4) | g;
-        move_particle(b, (int)b.vals[start + cstart[c] + lo], dst);
+        move_particle(b, (int)b.vals[start + cstart[c] + d], dst);
       }
     } else {
       for (int i = start + t; i < end; i += RO_T) {

@@ -111,10 +111,10 @@ def _layout_checks(sc, nblocks=300):
                 assert len(set(gc.tolist())) == 16
             extra.append(np.bincount(gc, minlength=16).max() - 1)
             left -= np.bincount(gc, minlength=16)
-        # inside a column: depth-major, cells in cx order
+        # inside a column: cell-major, cells in cx order, each in sorted order
         for g in range(16):
             sel = np.flatnonzero(cols == g)
-            order = depth[sel] * 4 + cx[sel]
+            order = cx[sel] * 4096 + depth[sel]
             assert np.all(np.diff(order) > 0)
     return float(np.mean(extra))
 
