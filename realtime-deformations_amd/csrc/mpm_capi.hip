@@ -453,7 +453,10 @@ int mpm_destroy(mpm_ctx* ctx) {
                   ctx->dmref};
   for (void* b : bufs) (void)hipFree(b);
   for (DevCollider* c : ctx->scoll) (void)hipFree(c);
-  for (float* c : ctx->adj_ckpt) (void)hipFree(c);
+  for (auto& k : ctx->adj_ckpt) {
+    void* b[] = {k.rows, k.pid, k.blist, k.nblk};
+    for (void* q : b) (void)hipFree(q);
+  }
   (void)hipFree(ctx->adj_const);
   (void)hipFree(ctx->adj_inv);
   for (auto& g : ctx->adj_segs) {
@@ -1103,15 +1106,47 @@ int adjoint_supported(mpm_ctx* ctx) {
   return MPM_OK;
 }
 
-// P rows [0, 33) and the constant rows back to the caller-ordered copies
-// (caller index = slot), ready to be binned.
-int restore_rows(mpm_ctx* ctx, const float* rows33) {
-  const int64_t sd = ctx->stride;
-  launch_iota(ctx->pid, ctx->n, ctx->stream);
-  MPM_HIP(hipMemcpyAsync(ctx->P, rows33, sizeof(float) * NA * sd, hipMemcpyDeviceToDevice, ctx->stream));
-  MPM_HIP(hipMemcpyAsync(ctx->P + (int64_t)FM * sd, ctx->adj_const, sizeof(float) * 3 * sd, hipMemcpyDeviceToDevice,
-                         ctx->stream));
-  ctx->since_sort = -1;
+// A checkpoint: the particle state with the binning it is in.  Saving bins
+// first, so the state is in a fresh binning's slot order; restoring puts
+// state and binning back, and the substeps that follow start without a
+// re-bin.
+int save_checkpoint(mpm_ctx* ctx, mpm_ctx::AdjCheckpoint& k) {
+  if (!k.rows) {
+    bool ok = hipMalloc(&k.rows, sizeof(float) * NFIELDS * ctx->stride) == hipSuccess &&
+              hipMalloc(&k.pid, sizeof(int) * (size_t)std::max(1, ctx->n)) == hipSuccess &&
+              hipMalloc(&k.blist, sizeof(int4) * (size_t)ctx->nblocks) == hipSuccess &&
+              hipMalloc(&k.nblk, sizeof(int)) == hipSuccess;
+    if (!ok) {
+      ctx->err = "device memory for the reverse-mode checkpoints is exhausted";
+      return MPM_ERR_NOMEM;
+    }
+  }
+  if (ctx->since_sort != 0)
+    if (int rc = rebin(ctx)) return rc;
+  hipStream_t s = ctx->stream;
+  MPM_HIP(hipMemcpyAsync(k.rows, ctx->P, sizeof(float) * NFIELDS * ctx->stride, hipMemcpyDeviceToDevice, s));
+  MPM_HIP(hipMemcpyAsync(k.pid, ctx->pid, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, s));
+  MPM_HIP(hipMemcpyAsync(k.blist, ctx->blist, sizeof(int4) * ctx->nblocks, hipMemcpyDeviceToDevice, s));
+  MPM_HIP(hipMemcpyAsync(k.nblk, ctx->nblk, sizeof(int), hipMemcpyDeviceToDevice, s));
+  return MPM_OK;
+}
+
+void free_checkpoints(mpm_ctx* ctx, size_t keep) {
+  while (ctx->adj_ckpt.size() > keep) {
+    const mpm_ctx::AdjCheckpoint& k = ctx->adj_ckpt.back();
+    void* b[] = {k.rows, k.pid, k.blist, k.nblk};
+    for (void* q : b) (void)hipFree(q);
+    ctx->adj_ckpt.pop_back();
+  }
+}
+
+int restore_checkpoint(mpm_ctx* ctx, const mpm_ctx::AdjCheckpoint& k) {
+  hipStream_t s = ctx->stream;
+  MPM_HIP(hipMemcpyAsync(ctx->P, k.rows, sizeof(float) * NFIELDS * ctx->stride, hipMemcpyDeviceToDevice, s));
+  MPM_HIP(hipMemcpyAsync(ctx->pid, k.pid, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, s));
+  MPM_HIP(hipMemcpyAsync(ctx->blist, k.blist, sizeof(int4) * ctx->nblocks, hipMemcpyDeviceToDevice, s));
+  MPM_HIP(hipMemcpyAsync(ctx->nblk, k.nblk, sizeof(int), hipMemcpyDeviceToDevice, s));
+  ctx->since_sort = 0;
   return MPM_OK;
 }
 
@@ -1265,19 +1300,19 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
   if (!ctx->adj_inv) MPM_HIP(hipMalloc(&ctx->adj_inv, sizeof(int) * (size_t)std::max(1, ctx->n)));
   launch_rows_to_caller(ctx->adj_const, ctx->P + (int64_t)FM * sd, ctx->pid, ctx->adj_inv, sd, ctx->n, 3, ctx->stream);
   const int nck = every == 1 ? 1 : (nsteps + every - 1) / every;
-  while ((int)ctx->adj_ckpt.size() > nck) {
-    (void)hipFree(ctx->adj_ckpt.back());
-    ctx->adj_ckpt.pop_back();
-  }
-  for (int c = 0; c < nck; ++c) {
-    if (c == (int)ctx->adj_ckpt.size()) {
-      float* buf = nullptr;
-      MPM_HIP(hipMalloc(&buf, sizeof(float) * NA * sd));
-      ctx->adj_ckpt.push_back(buf);
+  free_checkpoints(ctx, (size_t)nck);
+  ctx->adj_ckpt.resize((size_t)nck);
+  {
+    // with checkpoints, each segment runs from its checkpoint's binning
+    // without re-binning, as the reverse sweep re-runs it
+    const int keep_resort = ctx->resort_every;
+    if (every > 1) ctx->resort_every = std::max(keep_resort, every);
+    auto restore_resort = on_scope_exit([ctx, keep_resort] { ctx->resort_every = keep_resort; });
+    for (int c = 0; c < nck; ++c) {
+      if (int rc = save_checkpoint(ctx, ctx->adj_ckpt[c])) return rc;
+      if (every > 1)
+        if (int rc = mpm_substeps(ctx, dt, std::min(every, nsteps - c * every))) return rc;
     }
-    launch_rows_to_caller(ctx->adj_ckpt[c], ctx->P, ctx->pid, ctx->adj_inv, sd, ctx->n, NA, ctx->stream);
-    if (every > 1)
-      if (int rc = mpm_substeps(ctx, dt, std::min(every, nsteps - c * every))) return rc;
   }
   if (every == 1) {
     // keep every substep: segments of resort_every substeps, one binning each
@@ -1301,7 +1336,8 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
         }
       }
       const mpm_ctx::AdjSegment& g = ctx->adj_segs[i];
-      if (int rc = rebin(ctx)) return rc;
+      if (ctx->since_sort != 0)  // the first segment starts in the checkpoint's binning
+        if (int rc = rebin(ctx)) return rc;
       MPM_HIP(hipMemcpyAsync(g.pid, ctx->pid, sizeof(int) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
       MPM_HIP(hipMemcpyAsync(g.blist, ctx->blist, sizeof(int4) * ctx->nblocks, hipMemcpyDeviceToDevice, ctx->stream));
       MPM_HIP(hipMemcpyAsync(g.nblk, ctx->nblk, sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
@@ -1380,7 +1416,7 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
   const int nck = every == 1 ? 0 : (steps + every - 1) / every;
   for (int c = nck - 1; c >= 0 && !rc; --c) {
     const int ns = std::min(every, steps - c * every);
-    if ((rc = restore_rows(ctx, ctx->adj_ckpt[c])) || (rc = rebin(ctx))) break;
+    if ((rc = restore_checkpoint(ctx, ctx->adj_ckpt[c]))) break;
     // re-run the segment, keeping each substep's starting state and P2G sums
     if ((rc = record_segment(ctx, dt, ns, seg, gsave, glist, w.svd))) break;
     launch_rows_to_slots(Gs, Gc, ctx->pid, sd, n, NA, s);
@@ -1391,7 +1427,7 @@ int mpm_adjoint_backward(mpm_ctx* ctx, const float* gx, const float* gv, float* 
     ctx->gacc_clean = false;
   }
   ctx->resort_every = keep_resort;
-  if (!rc) rc = restore_rows(ctx, ctx->adj_ckpt[0]);  // leave the initial state
+  if (!rc) rc = restore_checkpoint(ctx, ctx->adj_ckpt[0]);  // leave the initial state
   ctx->gacc_clean = false;
   if (!rc) {
     MPM_HIP(hipGetLastError());

@@ -202,12 +202,19 @@ struct mpm_ctx {
   size_t events_used = 0;
   std::vector<int> event_kernel;    // kernel id per (start, stop) pair
   int64_t* dcount = nullptr;        // device counter scratch
-  // reverse mode: checkpoints of (x, v, B, F_E, F_P) in caller order every
-  // adj_every substeps of the last mpm_adjoint_forward, and the constant
+  // reverse mode: checkpoints every adj_every substeps of the last
+  // mpm_adjoint_forward (each the whole particle state in the slot order of a
+  // binning made at the checkpoint, with that binning), and the constant
   // fields (m, V0, E0) in caller order
   float adj_dt = 0.f;
   int adj_steps = 0, adj_every = 0;
-  std::vector<float*> adj_ckpt;
+  struct AdjCheckpoint {
+    float* rows = nullptr;  // NFIELDS rows, slot order
+    int* pid = nullptr;     // n: caller index of each slot
+    int4* blist = nullptr;  // nblocks: home-block list
+    int* nblk = nullptr;    // 1
+  };
+  std::vector<AdjCheckpoint> adj_ckpt;
   float* adj_const = nullptr;
   int* adj_inv = nullptr;  // slot of each caller index (row transfers to caller order)
   // checkpoint_every == 1: every substep's record kept, segment by segment

@@ -72,7 +72,8 @@ def _run(scs, steps, exact=True, tuning=None, batch=True, dt=1e-4, compose=M.COM
 
 
 def _max_rel(a, b):
-    return max(rel_err(a[i][f], b[i][f]) for i in range(len(a)) for f in FIELDS)
+    """(largest norm-relative difference, its scene and field)."""
+    return max((rel_err(a[i][f], b[i][f]), i, f) for i in range(len(a)) for f in FIELDS)
 
 
 def test_batch_equals_single_scenes_bit_for_bit():
@@ -96,7 +97,10 @@ def test_batch_equals_single_scenes_bit_for_bit():
     # for the record: the default mode's sums depend on the atomics' order
     dp, _, _ = _run(scs, steps, exact=False, tuning=(16, 2048))
     dq, _, _ = _run(scs, steps, exact=False, tuning=(8, 512), batch=False)
-    print(f"default mode, batch vs single after {steps} substeps: max norm-relative difference {_max_rel(dp, dq):.2e}")
+    e, i, f = _max_rel(dp, dq)
+    ex = max(rel_err(dp[k]["x"], dq[k]["x"]) for k in range(len(scs)))
+    print(f"default mode, batch vs single after {steps} substeps: largest norm-relative difference {e:.2e} "
+          f"(scene {i}, {f}); positions {ex:.2e}")
 
 
 def test_repeated_runs_are_bit_for_bit():
