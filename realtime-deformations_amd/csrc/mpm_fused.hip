@@ -422,13 +422,15 @@ __device__ __forceinline__ F3 advect(const KArgs& a, const Scene& S, F3 x, F3 v)
 // BATCHED = 0 (one scene): the scene comes from the kernel arguments,
 // read-only and loop-invariant; BATCHED = 1: each block's scene from the
 // device table, copied once per block.
-// LITERAL kernels are held to three waves per SIMD (<= 168 registers); the
-// TEXTBOOK ones, with their FP64 re-solve, would spill heavily there and are
-// held to two (<= 256 registers: the batched fused pass took 258 unbounded,
-// one wave per SIMD, and ran 1.5x slower).
+// Every variant is held to three waves per SIMD (<= 168 registers), the
+// occupancy the LDS halo allows.  In the TEXTBOOK variants most spilled
+// registers are saved around the out-of-line FP64 re-solve, which runs only
+// for near-degenerate spectra: at three waves their passes run 8-9% faster
+// than at two (the batched fused pass took 258 registers unbounded, one wave
+// per SIMD, and ran 1.5x slower).
 // EXACT: reproducible mode, the P2G contributions on the scene's quanta.
 template <int MODE, int COMPOSE, int BATCHED, int REC = 0, int EXACT = 0>
-__global__ void __launch_bounds__(TPB, COMPOSE == COMPOSE_LITERAL ? 3 : 2) k_block_pass(BlockArgs ba) {
+__global__ void __launch_bounds__(TPB, 3) k_block_pass(BlockArgs ba) {
   constexpr bool kG2P = MODE == MODE_FUSED || MODE == MODE_G2P;
   constexpr bool kP2G = MODE != MODE_G2P;
   __shared__ float vh[kG2P ? 3 * PLANE : 1];
