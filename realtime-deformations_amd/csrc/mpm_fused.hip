@@ -16,12 +16,13 @@
 // binning pass's list through a device counter: the grid velocities around
 // the block and the FP64 accumulators it writes live in LDS halos of 9x9x9
 // nodes (the block, the cubic stencil, and a one-cell margin for particles
-// that drifted since the last sort).  With halo strides (112, 12, 1) an FP64
-// atomic's bank pair depends on the node's (y, z) column; the binning's
-// column-group layout and a per-lane rotation of the stencil traversal
-// (lane_rotation) spread each 16-lane group over distinct pairs.  A particle
-// outside the margin takes a global-memory path.  The grid pass then turns
-// accumulators into velocities on the active blocks only.
+// that drifted since the last sort).  The accumulators are laid out in 4x4
+// (y, z) tiles (blk::tile_y, blk::tile_z), so an FP64 atomic's bank pair is
+// set by the node's (y mod 4, z mod 4), and a per-lane rotation of the
+// stencil traversal (lane_rotation) gives the 16 lanes of each atomic group
+// 16 distinct pairs.  A particle outside the margin takes a global-memory
+// path.  The grid pass then turns accumulators into velocities on the active
+// blocks only.
 //
 // Every block-list entry carries its scene: one launch steps all scenes of a
 // context, each with its own grid spacing, grid, material and colliders read
