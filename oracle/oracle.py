@@ -272,71 +272,13 @@ def bin_keys(x, h, dims):
     return (blk.astype(np.uint32) * np.uint32(64) + cell.astype(np.uint32)).astype(np.uint32)
 
 
-COL_RR_MAX, COL_GMAX = 64, 256
-
-
-def _block_layout(cells, srcs):
-    """Slot order of one block: caller indices per slot.  cells: cell in the
-    block (0..63) of the block's particles in stable sorted order, srcs: their
-    caller indices.  Column g = cell & 15 (cy, cz); a column's list is
-    cell-major: the particles of cell cx = cell >> 4 = 0 in sorted order,
-    then cx = 1, 2, 3.  Slots come in groups of 16: every column with
-    particles left gives one, spare slots go one at a time to the column with the most left
-    (lowest column on ties), columns in order inside a group.  A block with a
-    cell of more than COL_RR_MAX particles or more than COL_GMAX groups keeps
-    the plain round-robin: the r-th particle of cell c at
-    sum_c' min(cnt[c'], r) + #{c' < c : cnt[c'] > r}."""
-    nb = len(cells)
-    cnt = np.bincount(cells, minlength=64)
-    cstart = np.cumsum(cnt) - cnt
-    maxc = int(cnt.max())
-    if maxc > COL_RR_MAX or (nb + 15) // 16 > COL_GMAX:
-        r = np.arange(nb) - cstart[cells]
-        pos = np.zeros(nb, np.int64)
-        for cc in range(64):
-            k = cnt[cc]
-            pos += np.minimum(k, r) + ((cc < cells) & (k > r))
-        out = np.empty(nb, np.int64)
-        out[pos] = srcs
-        return out
-    col = []
-    for g in range(16):
-        col.append([((cx << 4) | g, d) for cx in range(4) for d in range(cnt[(cx << 4) | g])])
-    rem = [len(L) for L in col]
-    ptr = [0] * 16
-    out = []
-    left = nb
-    while left > 0:
-        size = min(16, left)
-        take = [1 if rem[g] > 0 else 0 for g in range(16)]
-        for _ in range(size - sum(take)):
-            best = max((g for g in range(16) if rem[g] - take[g] > 0), key=lambda g: (rem[g] - take[g], -g))
-            take[best] += 1
-        for g in range(16):
-            for q in range(take[g]):
-                c, d = col[g][ptr[g] + q]
-                out.append(srcs[cstart[c] + d])
-            ptr[g] += take[g]
-            rem[g] -= take[g]
-        left -= size
-    return np.array(out, np.int64)
-
-
 def bin_reference(x, h, dims):
-    """-> (sorted keys, stable permutation, caller index of each slot after
-    the in-block layout of _block_layout)."""
+    """-> (sorted keys, stable permutation, caller index of each slot).  The
+    particles take their slots in key order, so the slot's caller index is
+    the permutation itself."""
     key = bin_keys(x, h, dims)
     perm = np.argsort(key, kind="stable")
-    skey = key[perm]
-    n = key.shape[0]
-    blk = skey >> 6
-    cell = (skey & 63).astype(np.int64)
-    starts = np.flatnonzero(np.r_[True, blk[1:] != blk[:-1]])
-    ends = np.r_[starts[1:], n]
-    slot = np.empty(n, np.int64)
-    for s0, e0 in zip(starts, ends):
-        slot[s0:e0] = _block_layout(cell[s0:e0], perm[s0:e0])
-    return skey, perm.astype(np.uint32), slot
+    return key[perm], perm.astype(np.uint32), perm.astype(np.int64)
 
 
 # ------------------------------------------------------------ generators

@@ -7,10 +7,8 @@
 // sort (8-bit digits) orders (key, index) pairs exactly like a CPU stable
 // sort: ties keep index order.  The non-empty blocks are listed from
 // per-workgroup counts and a scan (no atomics), in a spread claim order
-// (claim_slot).  Each block's particles are then moved into one contiguous
-// range in column groups (see the layout note above k_reorder), so the lanes
-// of each 16-lane group start their stencils in different (y, z) columns and
-// the FP64 LDS atomics of P2G spread over the bank pairs.
+// (claim_slot), and the particles are moved into key order, so each block's
+// particles take one contiguous slot range, cell by cell.
 #include "mpm_bin.hpp"
 
 namespace mpm {
@@ -196,16 +194,15 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(const uint32_t* keys, const
   }
 }
 
-// First and last sorted index of every non-empty block; compact block list.
-// Block boundaries of the sorted keys: blk_start / blk_end per grid block,
-// and the number of blocks that start in each workgroup (wgcnt).
+// Block boundaries of the sorted keys: the end of every non-empty block's
+// slot range (blk_end, per grid block) and the number of blocks that start
+// in each workgroup (wgcnt).
 __global__ void __launch_bounds__(256) k_block_detect(BinArgs b, uint32_t* wgcnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   bool start = false;
   if (i < b.n) {
     const uint32_t blk = b.keys[i] >> 6;
     start = i == 0 || (b.keys[i - 1] >> 6) != blk;
-    if (start) b.blk_start[blk] = i;
     if (i == b.n - 1 || (b.keys[i + 1] >> 6) != blk) b.blk_end[blk] = i + 1;
   }
   const int c = __syncthreads_count(start);
@@ -274,8 +271,9 @@ __device__ __forceinline__ uint32_t claim_slot(uint32_t r, uint32_t nblk) {
   return (uint32_t)(((uint64_t)r * m) % nblk);
 }
 
-// Compact list of the non-empty blocks: the block starting in workgroup w
-// has rank wgoff[w] + its rank among that workgroup's starts in key order.
+// List of the non-empty blocks, (block, first slot, end slot, scene) each:
+// the block starting in workgroup w has rank wgoff[w] + its rank among that
+// workgroup's starts in key order, and list slot claim_slot(rank).
 __global__ void __launch_bounds__(256) k_block_ids(BinArgs b, const uint32_t* wgoff) {
   __shared__ uint32_t wcnt[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,141 +286,32 @@ __global__ void __launch_bounds__(256) k_block_ids(BinArgs b, const uint32_t* wg
     uint32_t r = wgoff[blockIdx.x];
     for (int w = 0; w < wave; ++w) r += wcnt[w];
     r += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    b.blk_ids[claim_slot(r, (uint32_t)*b.nblk)] = (int)(b.keys[i] >> 6);
+    const int blk = (int)(b.keys[i] >> 6);
+    b.blist[claim_slot(r, (uint32_t)*b.nblk)] = make_int4(blk, i, b.blk_end[blk], b.blk_scene[blk]);
   }
 }
 
-// Slot layout inside a block, chosen for the LDS banks of the block pass.
-// A particle's stencil origin in the halo sets the FP64 bank pair of all its
-// atomics by its cell's column (cy, cz); a wave's 64 lanes issue an atomic as
-// four 16-lane groups, each free of conflicts when its 16 lanes come from 16
-// different columns.  The slots are therefore cut into groups of 16, and
-// each group takes one particle from every column that still has particles;
-// when fewer than 16 columns remain, the group's spare slots go one by one
-// to the column with the most particles left (lowest column on ties).
-// Inside a group the columns come in order.  A column's k-th taken particle
-// is entry k of its list, which is cell-major: the particles of cell cx = 0
-// in sorted order, then cx = 1, 2, 3.  So the same lane of consecutive
-// groups mostly holds particles of one cell, and G2P reads of the two 16-lane
-// groups that issue together hit one address (a broadcast).
-//
-// Tables (one wave builds them): per group and column the first slot in the
-// group and the list position at the group's start.  Threads then walk
-// destination slots (coalesced writes) and find their sources by table
-// lookups.  Blocks with a cell of more than RR_MAX particles or more than
-// GMAX groups take the plain round-robin of the per-source formula instead
-// (slot of the r-th particle of cell c: sum_c' min(cnt[c'], r) +
-// #{c' < c : cnt[c'] > r}).
-constexpr int RR_MAX = 64;
-constexpr int GMAX = 256;
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
-}
-// Particle fields and caller index from slot src (current order) to slot
-// dst (binned order), in two halves whose loads are all issued before their
-// stores (half the fields at a time keeps the register count, and with it
-// the number of blocks being planned at once, up).
-__device__ __forceinline__ void move_particle(const BinArgs& b, int src, int dst) {
+// Particle fields and caller index of every slot moved into key order: slot
+// i takes the particle the sort put at i (vals[i], its slot in the current
+// order).  Particles move little between binnings, so the gathered loads
+// stay close to in order.  The loads of half the fields are all issued
+// before their stores.
+constexpr int RO_T = 256;
+__global__ void __launch_bounds__(RO_T) k_reorder(BinArgs b) {
   const float* __restrict__ P = b.P;
   float* __restrict__ Pout = b.Pout;
   constexpr int HALF = NFIELDS / 2;
+  for (int dst = blockIdx.x * RO_T + threadIdx.x; dst < b.n; dst += gridDim.x * RO_T) {
+    const int src = (int)b.vals[dst];
 #pragma unroll 1
-  for (int f0 = 0; f0 < NFIELDS; f0 += HALF) {
-    float v[HALF];
+    for (int f0 = 0; f0 < NFIELDS; f0 += HALF) {
+      float v[HALF];
 #pragma unroll
-    for (int f = 0; f < HALF; ++f) v[f] = P[(int64_t)(f0 + f) * b.stride + src];
+      for (int f = 0; f < HALF; ++f) v[f] = P[(int64_t)(f0 + f) * b.stride + src];
 #pragma unroll
-    for (int f = 0; f < HALF; ++f) Pout[(int64_t)(f0 + f) * b.stride + dst] = v[f];
-  }
-  b.pid_out[dst] = b.pid[src];
-}
-
-constexpr int RO_T = 256;
-__global__ void __launch_bounds__(RO_T) k_reorder(BinArgs b) {
-  __shared__ int cnt[64];
-  __shared__ int cstart[64];
-  __shared__ unsigned char gofs[GMAX][16];
-  __shared__ unsigned short gptr[GMAX][16];
-  __shared__ int table_s;
-  const int t = threadIdx.x;
-  const int nblk = *b.nblk;
-  for (int r = blockIdx.x; r < nblk; r += gridDim.x) {  // blocks in key order
-    const int s = (int)claim_slot((uint32_t)r, (uint32_t)nblk);
-    const int blk = b.blk_ids[s];
-    const int start = b.blk_start[blk], end = b.blk_end[blk];
-    const int nb = end - start;
-    if (t < 64) cnt[t] = 0;
-    __syncthreads();
-    for (int i = start + t; i < end; i += RO_T) atomicAdd(&cnt[b.keys[i] & 63u], 1);
-    __syncthreads();
-    if (t < 64) {
-      const int c = cnt[t];
-      int incl = c;
-      for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(incl, off, 64);
-        if (t >= off) incl += v;
-      }
-      cstart[t] = incl - c;
-      const int maxc = wave_max(c);
-      const bool table = maxc <= RR_MAX && (nb + 15) / 16 <= GMAX;
-      if (t == 0) table_s = table ? 1 : 0;
-      if (table) {
-        // groups of 16 slots (the whole wave runs the reductions; lanes 0..15
-        // hold column t, lanes >= 16 nothing)
-        int rem = t < 16 ? cnt[t] + cnt[16 | t] + cnt[32 | t] + cnt[48 | t] : 0, ptr = 0, left = nb;
-        for (int gi = 0; left > 0; ++gi) {
-          const int size = left < 16 ? left : 16;
-          int take = rem > 0 ? 1 : 0;
-          const int k = __popcll(__ballot(take != 0));
-          for (int e = size - k; e > 0; --e) {
-            const int key = (t < 16 && rem - take > 0) ? (((rem - take) << 6) | (63 - t)) : -1;
-            const int best = wave_max(key);
-            if (key >= 0 && key == best) ++take;
-          }
-          int pre = take;
-#pragma unroll
-          for (int off = 1; off < 16; off <<= 1) {
-            const int v = __shfl_up(pre, off, 64);
-            if ((t & 15) >= off) pre += v;
-          }
-          if (t < 16) {
-            gofs[gi][t] = (unsigned char)(pre - take);
-            gptr[gi][t] = (unsigned short)ptr;
-          }
-          ptr += take;
-          rem -= take;
-          left -= size;
-        }
-      }
+      for (int f = 0; f < HALF; ++f) Pout[(int64_t)(f0 + f) * b.stride + dst] = v[f];
     }
-    __syncthreads();
-    if (table_s) {
-      for (int dst = start + t; dst < end; dst += RO_T) {
-        const int j = dst - start, gi = j >> 4, r = j & 15;
-        int g = 15;
-        while (gofs[gi][g] > r) --g;
-        const int p = gptr[gi][g] + (r - gofs[gi][g]);
-        int cx = 0, d = p;  // entry p of the cell-major list: particle d of cell cx
-        while (cx < 3 && d >= cnt[(cx << 4) | g]) d -= cnt[(cx++ << 4) | g];
-        const int c = (cx << 4) | g;
-        move_particle(b, (int)b.vals[start + cstart[c] + d], dst);
-      }
-    } else {
-      for (int i = start + t; i < end; i += RO_T) {
-        const int c = (int)(b.keys[i] & 63u);
-        const int r = i - start - cstart[c];
-        int pos = 0;
-        for (int cc = 0; cc < 64; ++cc) {
-          const int k = cnt[cc];
-          pos += (k < r ? k : r) + ((cc < c && k > r) ? 1 : 0);
-        }
-        move_particle(b, (int)b.vals[i], start + pos);
-      }
-    }
-    if (t == 0) b.blist[s] = make_int4(blk, start, end, b.blk_scene[blk]);
-    __syncthreads();
+    b.pid_out[dst] = b.pid[src];
   }
 }
 
@@ -457,8 +346,7 @@ void launch_bin(BinArgs b, hipStream_t s) {
   k_block_detect<<<nb, 256, 0, s>>>(c, b.hist);
   k_block_scan<<<1, BS_T, 0, s>>>(b.hist, nb, b.nblk);
   k_block_ids<<<nb, 256, 0, s>>>(c, b.hist);
-  const int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (int64_t)b.n / 64), 4096);
-  k_reorder<<<grid, RO_T, 0, s>>>(c);
+  k_reorder<<<(b.n + RO_T - 1) / RO_T, RO_T, 0, s>>>(c);
 }
 
 // Keys and permutation only (the test hook for bit-exact sort parity).

@@ -17,9 +17,7 @@ struct BinArgs {
   int64_t nnodes;         // nodes over all scenes
   uint32_t *keys, *vals, *keys2, *vals2;  // 4 x n
   uint32_t* hist;                         // 256 x ceil(n / 4096)
-  int* blk_start;                         // per grid block
-  int* blk_end;
-  int* blk_ids;                           // compact list of non-empty blocks
+  int* blk_end;                           // per grid block: end of its slot range
   int4* blist;                            // (block, start, end, scene) per listed block
   int* nblk;                              // number of listed blocks
 };

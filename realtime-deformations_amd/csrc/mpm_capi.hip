@@ -393,9 +393,7 @@ int mpm_create_batch(const mpm_params* params, const int32_t* n_particles, int32
   alloc(&ctx->keys2, sizeof(uint32_t) * nn);
   alloc(&ctx->vals2, sizeof(uint32_t) * nn);
   alloc(&ctx->hist, sizeof(uint32_t) * 256 * (size_t)(std::max<int64_t>(1, ntiles) + 1));
-  alloc(&ctx->blk_start, sizeof(int) * nb);
   alloc(&ctx->blk_end, sizeof(int) * nb);
-  alloc(&ctx->blk_ids, sizeof(int) * nb);
   alloc(&ctx->blist, sizeof(int4) * nb);
   alloc(&ctx->nblk, sizeof(int));
   alloc(&ctx->gflag, sizeof(int) * nb);
@@ -447,7 +445,7 @@ int mpm_destroy(mpm_ctx* ctx) {
   for (hipGraphExec_t& g : ctx->gexec)
     if (g) (void)hipGraphExecDestroy(g);
   void* bufs[] = {ctx->P,       ctx->P2,      ctx->pid,      ctx->pid2,  ctx->keys,     ctx->vals,     ctx->keys2,
-                  ctx->vals2,   ctx->hist,    ctx->blk_start, ctx->blk_end, ctx->blk_ids, ctx->blist,   ctx->nblk,
+                  ctx->vals2,   ctx->hist,    ctx->blk_end, ctx->blist,   ctx->nblk,
                   ctx->gflag,   ctx->glist[0], ctx->glist[1], ctx->gcnt,  ctx->dstep,    ctx->gacc,     ctx->gvel,
                   ctx->gforce,  ctx->status,  ctx->dcount,  ctx->d_scenes, ctx->blk_scene, ctx->bctr,
                   ctx->dmref};
@@ -657,9 +655,7 @@ BinArgs bin_args(mpm_ctx* ctx) {
   b.keys2 = ctx->keys2;
   b.vals2 = ctx->vals2;
   b.hist = ctx->hist;
-  b.blk_start = ctx->blk_start;
   b.blk_end = ctx->blk_end;
-  b.blk_ids = ctx->blk_ids;
   b.blist = ctx->blist;
   b.nblk = ctx->nblk;
   return b;

@@ -162,7 +162,7 @@ struct mpm_ctx {
   int* pid2 = nullptr;
   // binning
   uint32_t *keys = nullptr, *vals = nullptr, *keys2 = nullptr, *vals2 = nullptr, *hist = nullptr;
-  int *blk_start = nullptr, *blk_end = nullptr, *blk_ids = nullptr, *nblk = nullptr;
+  int *blk_end = nullptr, *nblk = nullptr;
   int4* blist = nullptr;
   // active grid blocks of the fused path
   int* gflag = nullptr;

@@ -208,9 +208,9 @@ def test_warm_state_one_substep():
 
 @pytest.mark.parametrize("which", ["reference", "snowball", "two_snowballs", "random", "ragged_slab", "clumped"])
 def test_binning_matches_cpu_stable_sort(which):
-    """Keys and permutation of the radix sort equal numpy's stable argsort;
-    the in-block slot layout (column groups, and the plain round-robin for a
-    block with a cell of more than 64 particles) equals its CPU definition."""
+    """Keys and permutation of the radix sort equal numpy's stable argsort,
+    and the particles take their slots in that order (a clumped cell of 300
+    particles included)."""
     if which == "reference":
         sc = scenes.reference_scene()
     elif which == "snowball":
