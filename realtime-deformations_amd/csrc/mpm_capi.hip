@@ -629,6 +629,7 @@ BlockArgs block_args(mpm_ctx* ctx, float dt) {
   ba.glist[1] = ctx->glist[1];
   ba.gcnt = ctx->gcnt;
   ba.dstep = ctx->dstep;
+  ba.advance = 0;
   ba.nblocks = ctx->nblocks;
   ba.grid = ctx->block_grid;
   ba.rec = nullptr;
@@ -685,18 +686,19 @@ void reset_grid(mpm_ctx* ctx) {
 }
 
 // One substep: re-bin when due, then the block pass (PASS_P2G on the first
-// substep of a call, PASS_FUSED after it), the grid pass and the epoch advance.
+// substep of a call, PASS_FUSED after it), which advances the epoch at its
+// end, and the grid pass.
 int substep(mpm_ctx* ctx, float dt, bool first) {
   int rc;
   if (ctx->since_sort < 0 || ctx->since_sort >= ctx->resort_every)
     if ((rc = rebin(ctx))) return rc;
-  const BlockArgs ba = block_args(ctx, dt);
+  BlockArgs ba = block_args(ctx, dt);
+  ba.advance = 1;
   hipStream_t s = ctx->stream;
   if ((rc = timed(ctx, first ? MPM_KERNEL_P2G : MPM_KERNEL_FUSED,
                   [&] { launch_block_pass(ba, first ? PASS_P2G : PASS_FUSED, s); })) ||
       (rc = timed(ctx, MPM_KERNEL_GRID, [&] { launch_grid_active(ba, s); })))
     return rc;
-  launch_advance_step(ctx->dstep, ctx->gcnt, s);
   ++ctx->since_sort;
   return MPM_OK;
 }
@@ -1171,9 +1173,9 @@ int record_segment(mpm_ctx* ctx, float dt, int ns, float* seg, float4* gs, int* 
     ba.rec = seg + (size_t)k * NA * sd;
     ba.rec_svd = sv + (size_t)k * SVD_ROWS * sd;
     ba.grec = grid_record(ctx, gs, gl, nullptr, k);
+    ba.advance = 1;
     launch_block_pass(ba, k == 0 ? PASS_P2G : PASS_FUSED, s);
     launch_grid_active(ba, s);
-    launch_advance_step(ctx->dstep, ctx->gcnt, s);
     ++ctx->since_sort;
   }
   BlockArgs ba = block_args(ctx, dt);

@@ -440,7 +440,14 @@ __global__ void __launch_bounds__(TPB, 3) k_block_pass(BlockArgs ba) {
   const KArgs& a = ba.a;
   const int t = threadIdx.x;
   const int nblocks = *ba.nblk;
-  if ((int)blockIdx.x >= nblocks) return;
+  if ((int)blockIdx.x >= nblocks) {
+    if (ba.advance && nblocks == 0 && blockIdx.x == 0 && t == 0) {  // nothing to claim
+      const int step = *ba.dstep;
+      *ba.dstep = step + 1;
+      ba.gcnt[(step + 1) % 3] = 0;
+    }
+    return;
+  }
   for (int s; (s = claim_block(ba.bctr, nblocks, &claimed)) >= 0;) {
     const int4 e = ba.blist[s];
     const int scene = e.w, start = e.y, end = e.z;
@@ -541,6 +548,15 @@ __global__ void __launch_bounds__(TPB, 3) k_block_pass(BlockArgs ba) {
       if (t < 27) mark_block(ba, S, bi - 1 + t / 9, bj - 1 + (t / 3) % 3, bk - 1 + t % 3);
     }
     __syncthreads();
+  }
+  // The workgroup whose claim is the last of the launch (each other one has
+  // made its final claim after finishing its blocks) advances the epoch for
+  // the grid pass and the next substep, and clears the block counter the
+  // next substep fills (last used two substeps ago).
+  if (ba.advance && t == 0 && claimed == nblocks + ((int)gridDim.x < nblocks ? (int)gridDim.x : nblocks) - 1) {
+    const int step = *ba.dstep;
+    *ba.dstep = step + 1;
+    ba.gcnt[(step + 1) % 3] = 0;
   }
 }
 
@@ -762,7 +778,7 @@ template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(Bloc
   const KArgs& a = ba.a;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int step = *ba.dstep;
+  const int step = *ba.dstep - 1;  // the epoch of the block pass just run
   const int* glist_cur = ba.glist[step & 1];
   const int* glist_prev = ba.glist[(step + 1) & 1];
   const int ncur = ba.gcnt[step % 3];
@@ -816,17 +832,8 @@ template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(Bloc
   }
 }
 
-// Next substep: advance the epoch and clear the counter it will fill (last
-// used two substeps ago).
-__global__ void k_advance_step(int* dstep, int* gcnt) {
-  const int s = *dstep;
-  *dstep = s + 1;
-  gcnt[(s + 1) % 3] = 0;
-}
-
 }  // namespace
 
-void launch_advance_step(int* dstep, int* gcnt, hipStream_t s) { k_advance_step<<<1, 1, 0, s>>>(dstep, gcnt); }
 
 template <int BATCHED> void launch_block_pass_t(const BlockArgs& ba, int mode, hipStream_t s) {
   const int grid = ba.grid;

@@ -4,10 +4,10 @@
 
 namespace mpm {
 
-// The substep epoch lives on the device (*dstep, advanced by
-// launch_advance_step after each grid pass), so one captured substep can be
-// replayed as a HIP graph: grid-block lists alternate with the epoch's
-// parity, their counters rotate with it modulo 3.
+// The substep epoch lives on the device (*dstep, advanced at the end of each
+// substep's block pass; the grid pass that follows works on epoch *dstep - 1),
+// so one captured substep can be replayed as a HIP graph: grid-block lists
+// alternate with the epoch's parity, their counters rotate with it modulo 3.
 // Reverse-mode record of one substep's grid: the P2G sums of its active
 // grid blocks, 64 nodes each, in the order of list (list[0]: the count,
 // list[1 + s]: the grid block of record slot s).  While the substep is
@@ -26,7 +26,8 @@ struct BlockArgs {
   int* gflag;         // per grid block: epoch of its last activation
   int* glist[2];      // grid blocks activated, by epoch parity
   int* gcnt;          // [3] counters, by epoch modulo 3
-  const int* dstep;   // substep epoch (> 0)
+  int* dstep;         // substep epoch (> 0)
+  int advance;        // block pass: advance the epoch once every workgroup is done
   int64_t nblocks;    // grid blocks over all scenes
   int grid;           // workgroups of the persistent block pass
   int* bctr;          // home-block claim counter (0 between launches)
@@ -56,6 +57,5 @@ void launch_adj_g2p_block(const BlockArgs& ba, const float* St, const float* Xn,
                           float4* gnode, const float4* vnode, hipStream_t s);
 void launch_block_pass(const BlockArgs& ba, int mode, hipStream_t s);
 void launch_grid_active(const BlockArgs& ba, hipStream_t s);
-void launch_advance_step(int* dstep, int* gcnt, hipStream_t s);
 
 }  // namespace mpm
