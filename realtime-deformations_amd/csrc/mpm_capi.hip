@@ -1114,7 +1114,10 @@ int save_checkpoint(mpm_ctx* ctx, mpm_ctx::AdjCheckpoint& k) {
               hipMalloc(&k.pid, sizeof(int) * (size_t)std::max(1, ctx->n)) == hipSuccess &&
               hipMalloc(&k.blist, sizeof(int4) * (size_t)ctx->nblocks) == hipSuccess &&
               hipMalloc(&k.nblk, sizeof(int)) == hipSuccess;
-    if (!ok) {
+    if (!ok) {  // all or nothing: a later call allocates again
+      void* b[] = {k.rows, k.pid, k.blist, k.nblk};
+      for (void* q : b) (void)hipFree(q);
+      k = mpm_ctx::AdjCheckpoint{};
       ctx->err = "device memory for the reverse-mode checkpoints is exhausted";
       return MPM_ERR_NOMEM;
     }
