@@ -1288,7 +1288,8 @@ int mpm_adjoint_forward(mpm_ctx* ctx, float dt, int32_t nsteps, int32_t every) {
     size_t free_b = 0, total_b = 0;
     const double per_step = (double)sizeof(float) * (NA + SVD_ROWS) * sd + (double)sizeof(float4) * ctx->nnodes +
                             (double)sizeof(int) * (ctx->nblocks + 1);
-    const double need = per_step * (nsteps + (double)seg_ns.size()) + (double)sizeof(float) * NA * sd;
+    const double need = per_step * (nsteps + (double)seg_ns.size()) + (double)sizeof(float) * NFIELDS * sd +
+                        (double)sizeof(int) * ctx->n + (double)sizeof(int4) * ctx->nblocks;  // + the checkpoint
     every = (same && !ctx->adj_segs.empty()) ||
                     (hipMemGetInfo(&free_b, &total_b) == hipSuccess && need < 0.5 * (double)free_b)
                 ? 1
