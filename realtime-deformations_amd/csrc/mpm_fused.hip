@@ -801,8 +801,9 @@ template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(Bloc
   const int ncur = ba.gcnt[step % 3];
   const int nprev = ba.gcnt[(step + 2) % 3];
   if (ba.grec.acc && blockIdx.x == 0 && threadIdx.x == 0) ba.grec.list[0] = ncur;
-  for (int s = blockIdx.x * 4 + wave; s < ncur + nprev; s += gridDim.x * 4) {
-    if (s < ncur) {
+  const int wid = blockIdx.x * 4 + wave, nwaves = gridDim.x * 4;
+  for (int s = wid; s < ncur; s += nwaves) {
+    {
       const int gb = glist_cur[s];
       const int64_t g = (int64_t)gb * 64 + lane;
       // the sums as FP32, as the grid update uses them; the accumulators
@@ -842,9 +843,16 @@ template <int BATCHED> __global__ void __launch_bounds__(TPB) k_grid_active(Bloc
         out = make_float4(v.x, v.y, v.z, acc.x);
       }
       a.gvel[g] = out;
-    } else {
-      const int gb = glist_prev[s - ncur];
-      if (ba.gflag[gb] != step) a.gvel[(int64_t)gb * 64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+  // Blocks active in the previous substep but not in this one get zero
+  // velocity.  One lane checks one block: most are still active, so a wave
+  // per block would mostly idle.
+  for (int s = wid * 64 + lane; s < nprev; s += nwaves * 64) {
+    const int gb = glist_prev[s];
+    if (ba.gflag[gb] != step) {
+      float4* gv = a.gvel + (int64_t)gb * 64;
+      for (int k = 0; k < 64; ++k) gv[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
 }
