@@ -189,6 +189,37 @@ def test_snowball_free_fall_closed_form_on_the_device():
     g.close()
 
 
+def test_nodes_that_lost_their_mass_read_zero():
+    """The falling snowball leaves grid blocks behind every few substeps.
+    After 150 substeps the dense grid holds mass and velocity only where
+    the particles of substep 149 deposit: nodes that had mass at substep 50
+    and have none now read zero mass and zero velocity (the grid pass clears
+    blocks that dropped out of the active list), and the mass equals the
+    oracle's P2G of the positions the last substep started from."""
+    sc = scenes.snowball()
+    g, _ = _pair(sc)
+    g.init_volumes()
+    dt = sc["dt"]
+    g.substeps(dt, 50)
+    early = g.grid()
+    g.substeps(dt, 99)
+    P = g.particles()
+    g.substeps(dt, 1)
+    late = g.grid()
+    o = O.Sim(O.make_params(sc["dims"], sc["h"], bc_mode=sc["params"]["bc_mode"], theta_s=sc["params"]["theta_s"],
+                            threads=1), P["x"].shape[0])
+    o.set_particles(P["x"], P["v"], sc["m"], sc["E0"])
+    o.stage(O.STAGE_P2G)
+    want = o.grid()["m"]
+    empty = want == 0
+    left = (early["m"] > 0) & empty
+    assert left.sum() > 100  # blocks did drop out between substeps 50 and 150
+    assert np.all(late["m"][empty] == 0.0)
+    assert np.all(late["vel"][empty] == 0.0)
+    assert rel_err(late["m"], want) < 1e-5
+    g.close()
+
+
 def test_warm_state_one_substep():
     """A warm state (non-zero B, F_E off identity, F_P != I) and one fused
     substep: exercises stress, hardening and the clamp."""
