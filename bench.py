@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the MPM snow substep on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-autotune]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-autotune] [--no-secondary]
                     [--workload slab4m|slab4m_landing|batch64|adjoint16]
 
 Workload: the 4,194,304-particle snow slab on a 256^3 grid (dx = 1/256,
@@ -29,6 +29,10 @@ cpu_baseline: the oracle (a CPU restatement of the reference path) timed on
 this host on the same slab: serial like the reference, and on all the host
 cores this process may use, with the calibration to the reference's own
 speed per core.
+batch64, adjoint16: the secondary workloads, measured in the same run as
+their own --workload runs measure them (64 scenes batched against one at a
+time, 500 substeps; forward + reverse mode, 200 substeps); --no-secondary
+leaves them out.
 """
 from __future__ import annotations
 
@@ -78,6 +82,12 @@ NFIELDS_BYTES = 36 * 4
 def substep_alg_bytes(a_over_n: float) -> float:
     """B_alg = 2(12+12+36+36+36) + 4*3 + 56 A/N per particle-substep."""
     return 2 * (12 + 12 + 36 + 36 + 36) + 4 * 3 + 56 * a_over_n
+
+
+def _note(msg):
+    """A progress line on stderr after each stage (stdout holds only the
+    JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def _host_threads():
@@ -286,6 +296,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="slab4m: time the library's default tuning instead of the autotuner's choice")
     ap.add_argument("--no-landing", action="store_true", help="slab4m: skip the deforming landing slab")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="slab4m: skip the batch64 and adjoint16 records (each is also its own --workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {args.gpus}); "
@@ -294,9 +306,11 @@ def main():
     if args.steps is None:
         args.steps = 500 if args.workload == "batch64" else 200
     if args.workload == "batch64":
-        return bench_batch(args)
+        print(json.dumps(bench_batch(args)), flush=True)
+        return
     if args.workload == "adjoint16":
-        return bench_adjoint(args)
+        print(json.dumps(bench_adjoint(args)), flush=True)
+        return
 
     import mpm_amd as M
     from mpm_amd import scenes
@@ -331,6 +345,7 @@ def main():
     resort, _ = sim.tuning()
     warm_run = aligned_warmup(sim, dt, args.warmup)
     wall_ms, gpu_ms, direct_ms, kern = timed_substeps(sim, dt, args.steps)
+    _note(f"slab4m: {args.steps} substeps timed")
     active = sim.count_active()
     a_over_n = active / n
     ms_per_step = wall_ms / args.steps
@@ -374,10 +389,19 @@ def main():
     if args.copy_probe:
         sim.copy_probe(1 << 30)
     sim.close()
+    _note("slab4m: reproducible mode and setup/readback timed")
     if not args.no_landing:
         out["deforming"] = bench_landing(M, args, args.steps, args.warmup)
+        _note("slab4m_landing: timed")
+    if not args.no_secondary:
+        # the two secondary workloads at their own sizes and substep counts
+        out["batch64"] = secondary(bench_batch, 500, args.warmup)
+        _note("batch64: timed")
+        out["adjoint16"] = secondary(bench_adjoint, 200, args.warmup)
+        _note("adjoint16: timed")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, V0, args.cpu_threads or _host_threads())
+        _note("cpu_baseline: timed")
     print(json.dumps(out), flush=True)
 
 
@@ -512,7 +536,7 @@ def bench_batch(args):
     }
     if tuned:
         out["autotune"] = tuned
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_adjoint(args):
@@ -630,7 +654,18 @@ def bench_adjoint(args):
         "status": status,
         "grad_finite": bool(np.isfinite(gx0).all() and np.isfinite(gv0).all() and np.isfinite(gE0).all()),
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+_CONTRACT_ONLY = ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "dtype", "data")
+
+
+def secondary(fn, steps, warmup):
+    """A secondary workload's record inside the headline line: the same
+    measurement as its own --workload run, without the keys the line
+    already states."""
+    rec = fn(argparse.Namespace(steps=steps, warmup=warmup, autotune=False))
+    return {k: v for k, v in rec.items() if k not in _CONTRACT_ONLY}
 
 
 def _profiled(name, workload, kernel, key):
