@@ -192,3 +192,42 @@ def test_reverse_mode_in_reproducible_mode():
             assert np.isfinite(x).all()
             err = np.linalg.norm(x.astype(np.float64) - r) / np.linalg.norm(r.astype(np.float64))
             assert err < tol, (every, err, tol)
+
+
+def test_profiled_launches_match_graph_replays():
+    """The per-kernel profile (mpm_profile_begin with per_kernel = 1) launches
+    every kernel directly between HIP events instead of replaying the
+    substep graphs.  In reproducible mode it gives the same bits as the
+    graph replays; it reports one first-substep P2G pass, a fused pass for
+    each later substep, a grid pass per substep, the closing G2P pass and at
+    least one re-bin, with positive times; and mpm_count_active counts the
+    grid nodes that hold mass."""
+    scs = _mixed_scenes()
+    steps, dt = 40, 1e-4
+    ref, ref_grids, _ = _run(scs, steps, tuning=(16, 1024))
+    sim = M.Simulation([_params(s) for s in scs], [s["x"].shape[0] for s in scs])
+    sim.set_reproducible(True)
+    for k, s in enumerate(scs):
+        sim.scene_set_particles(k, s["x"], s["v"], s["m"], s["E0"])
+    sim.init_volumes()
+    sim.set_tuning(16, 1024)
+    sim.profile_begin(per_kernel=True)
+    sim.substeps(dt, steps)
+    total_ms, kern = sim.profile_end()
+    sim.sync()
+    for k in range(len(scs)):
+        P = sim.scene_particles(k)
+        for f in FIELDS:
+            assert np.array_equal(P[f], ref[k][f]), (k, f)
+        G = sim.scene_grid(k)
+        assert np.array_equal(G["m"], ref_grids[k]["m"]), k
+    assert kern["p2g"][1] == 1 and kern["fused"][1] == steps - 1
+    assert kern["grid"][1] == steps and kern["g2p"][1] == 1
+    assert kern["bin"][1] >= 1
+    for name in ("p2g", "fused", "grid", "g2p", "bin"):
+        assert kern[name][0] > 0.0, name
+    assert total_ms >= kern["fused"][0]
+    # the dense grids of all scenes, counted on the device and on the host
+    with_mass = sum(int(np.count_nonzero(sim.scene_grid(k)["m"])) for k in range(len(scs)))
+    assert sim.count_active() == with_mass > 0
+    sim.close()
