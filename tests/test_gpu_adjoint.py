@@ -305,9 +305,9 @@ def test_fit_E0_and_v0_to_a_target_trajectory():
         loss = ((xT - xT_target) ** 2).sum(dim=1).mean() / float(sc["h"]) ** 2
         loss.backward()
         opt.step()
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     dv0 = float(np.linalg.norm(v0.detach().numpy() - np.float32([0.3, -1.0, 0.2])))
-    print(f"fit: loss {losses[0]:.4e} -> {losses[-1]:.4e}, |v0 - v0*| {dv0:.4f}, E {float(torch.exp(logE)):.4e}")
+    print(f"fit: loss {losses[0]:.4e} -> {losses[-1]:.4e}, |v0 - v0*| {dv0:.4f}, E {float(torch.exp(logE.detach())):.4e}")
     assert losses[-1] < 0.1 * losses[0]
     assert dv0 < 0.2 * float(np.linalg.norm([0.3, 0.0, 0.2]))
     g.close()
